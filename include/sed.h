@@ -1,0 +1,158 @@
+/*
+ * sed.h -- C ABI of the MI355X (gfx950) matrix-free symmetry-sector solver.
+ *
+ * Drop-in for the matrix-free mode of GrothendieckUniverse/RealSpace_ExactDiagonalization:
+ * a host shim keeps the reference's model / symmetry-group / sector-basis types and
+ * forwards the operator calls here.  Every entry point below names the reference
+ * interface it replaces (src/symmetry_resolved_ed.jl unless stated otherwise).
+ *
+ * Conventions (the reference's Julia layout):
+ *   - complex numbers are two contiguous doubles (re, im), i.e. Julia ComplexF64;
+ *   - site indices are 1-based, bit (i-1) of a mask is site i;
+ *   - group perm[g*n_site + (i-1)] is the 1-based target of site i under element g;
+ *   - table indices (combinadic ranks, orbit indices, group indices) are 1-based
+ *     where the reference's are.
+ * Ownership: host buffers are borrowed for the duration of the call and never
+ * retained; device memory belongs to the handles and is released by *_destroy.
+ * Errors: every function returns SED_OK (0) or a negative code; the message is in
+ * sed_last_error() (thread-local).  No C++ exception crosses this ABI.
+ * Threading: a handle is not thread-safe; distinct handles on distinct host
+ * threads are independent.  Calls are synchronous with respect to host buffers.
+ */
+#ifndef SED_H
+#define SED_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SED_OK 0
+#define SED_EINVAL (-1)       /* bad argument */
+#define SED_ENOMEM (-2)       /* device allocation failed / over budget */
+#define SED_EDEVICE (-3)      /* HIP runtime error */
+#define SED_ENOCONV (-4)      /* eigensolver did not converge */
+#define SED_EUNSUPPORTED (-5) /* input outside what the device path handles */
+#define SED_ESTATE (-6)       /* call out of order (e.g. matvec before set_terms) */
+
+#define SED_BOSONIC 0
+#define SED_FERMIONIC 1
+
+/* arithmetic of a matvec / solve */
+#define SED_PREC_FP64 0        /* complex FP64 vectors and matvec */
+#define SED_PREC_FP32 1        /* complex FP32 vectors and matvec, FP32 accumulation */
+#define SED_PREC_FP32_REFINE 2 /* FP32 Lanczos, then FP64 Lanczos restarted from its Ritz vectors */
+
+typedef struct sed_context sed_context;
+typedef struct sed_catalog sed_catalog;
+typedef struct sed_sector sed_sector;
+typedef struct sed_server sed_server;
+
+typedef struct sed_solve_stats {
+    int32_t converged;        /* number of converged eigenpairs */
+    int32_t iters_fp32;       /* Lanczos steps in FP32 */
+    int32_t iters_fp64;       /* Lanczos steps in FP64 */
+    int32_t restarts;         /* thick restarts */
+    int64_t matvecs;          /* H|psi> applications */
+    double residual_max;      /* max residual norm of the returned pairs */
+    double seconds;           /* wall time of the solve */
+} sed_solve_stats;
+
+/* Thread-local message of the last failing call on this thread. */
+const char* sed_last_error(void);
+/* ABI version of this library (major*10000 + minor*100 + patch). */
+int sed_version(void);
+
+/* One context per GPU (device ordinal), owning a HIP stream.  No reference
+ * counterpart: the reference runs on CPU threads. */
+int sed_create(int device, sed_context** out);
+int sed_destroy(sed_context* ctx);
+
+/* build_symmetry_orbit_catalog(; n_filled, symmetry_group, particle_statistics)
+ * (:371-487) together with Finite_Symmetry_Group (:90-104) and
+ * _inverse_operation_indices (:109-117).  The site phases must be of gauge form
+ * eta_g(i) = f(i)/f(perm_g(i)) (true of build_translation_group at any twist,
+ * :1374-1405); otherwise SED_EUNSUPPORTED.  Requires n_site <= 64,
+ * C(n_site, n_filled) < 2^31 and n_group <= 64. */
+int sed_catalog_build(sed_context* ctx, int n_site, int n_filled, int statistics,
+                      int n_group, int identity_idx, const int32_t* perm, const double* phases,
+                      sed_catalog** out);
+/* length(rep_rank_table) and length(representative_mask_list) of the catalog. */
+int sed_catalog_info(const sed_catalog* cat, int64_t* n_total, int64_t* n_orbits);
+/* Julia-layout dump of Symmetry_Orbit_Catalog (:305-327): rep_rank_table (1-based
+ * orbit index), rep_sym_table (1-based inverse group index), rep_amp_table
+ * (ComplexF64), representative_mask_list and stabilizer_order_list.  Any pointer
+ * may be NULL.  Arrays are n_total / n_orbits long. */
+int sed_catalog_copy(const sed_catalog* cat, int32_t* rep_rank, int32_t* rep_sym, double* rep_amp,
+                     uint64_t* reps, int64_t* stab_order);
+/* stabilizer_g_indices_list / stabilizer_phases_list (:309-310) as padded
+ * n_orbits x n_group arrays (1-based indices, 0 = padding). */
+int sed_catalog_stabilizers(const sed_catalog* cat, int32_t* stab_gidx, double* stab_phase);
+int sed_catalog_destroy(sed_catalog* cat);
+
+/* build_symmetry_sector_basis(catalog, irrep; atol) (:544-556) with the irrep's
+ * characters chi (n_group ComplexF64, OneDim_Irrep.values :501-504).  The catalog
+ * must outlive the sector. */
+int sed_sector_build(sed_catalog* cat, const double* chi, double atol, sed_sector** out, int64_t* dim);
+/* representative_mask_list and stabilizer_order_list of the sector basis (:522-528). */
+int sed_sector_basis(const sed_sector* sec, uint64_t* reps, int64_t* stab_order);
+/* hamiltonian_linear_operator(basis, bilinear_terms, density_terms, cmap)
+ * (:891-897) / MatrixFreeHamiltonian(...) (:779-791): attaches the term lists
+ * (i_from, i_to, t) and (i, j, V), 1-based sites, t and V ComplexF64. */
+int sed_sector_set_terms(sed_sector* sec, int n_bilinear, const int32_t* i_from, const int32_t* i_to,
+                         const double* t, int n_density, const int32_t* di, const int32_t* dj,
+                         const double* v);
+/* Exact hop counts of one matvec: gated hops (occupancy gate passed, one rep-table
+ * read each) and in-sector hops (one x gather each); they define the algorithmic
+ * bytes of the H|psi> kernel. */
+int sed_sector_hop_counts(sed_sector* sec, int64_t* hops_gated, int64_t* hops_in_sector);
+/* (H::MatrixFreeHamiltonian)(x) (:880-884) / _apply_table! (:731-765): y = H x for
+ * host vectors of sector_dim ComplexF64; precision SED_PREC_FP64 or SED_PREC_FP32. */
+int sed_matvec(sed_sector* sec, const double* x, double* y, int precision);
+/* Same on device buffers of sector_dim complex elements (double2 for FP64, float2
+ * for FP32) in the sector basis; ordered on the context's stream. */
+int sed_matvec_device(sed_sector* sec, const void* x_dev, void* y_dev, int precision);
+/* Time n_calls device matvecs on resident random vectors; returns the mean
+ * kernel time in ms measured with HIP events on the context's stream. */
+int sed_matvec_benchmark(sed_sector* sec, int precision, int n_calls, double* ms_per_call);
+/* diagonalize_block_matrixfree(H_op, n; nev) (:998-1030): lowest nev eigenvalues
+ * (ascending) and optionally eigenvectors (sector_dim x nev column-major ComplexF64,
+ * may be NULL).  Sectors of dim <= 500 are solved densely, as the reference does;
+ * larger ones by thick-restart Lanczos with full reorthogonalisation from a seeded
+ * Gaussian start vector, residual tolerance tol (reference: 1e-10), at most maxiter
+ * matvecs per precision phase.  Returns SED_ENOCONV if fewer than nev converge. */
+int sed_solve(sed_sector* sec, int nev, double tol, uint64_t seed, int maxiter, int precision,
+              double* evals, double* evecs, sed_solve_stats* stats);
+/* Site densities <n_i> of eigenvector `which` (0-based) of the last sed_solve on this
+ * sector, n_site doubles (observables/static_structure_factor.jl:62-83 convention). */
+int sed_sector_density(sed_sector* sec, int which, double* density);
+int sed_sector_destroy(sed_sector* sec);
+
+/* Continuous-batching service of many sector solves (no reference counterpart; it
+ * replaces the per-sector loop of ed_scan! / _ed_scan_sectors! :1199-1356 for the
+ * flux scan).  Requests are admitted while their HBM footprint fits the budget
+ * (hbm_budget_bytes <= 0: 80% of free memory) and one of max_slots solver lanes is
+ * free; each lane has its own HIP stream, so admitted requests overlap on the
+ * device, and a finished request frees its lane for the queue at once.  Two requests
+ * on the same sector object never run at the same time. */
+int sed_server_create(sed_context* ctx, int64_t hbm_budget_bytes, int max_slots, sed_server** out);
+/* Queue one solve of `sec` (which must stay alive until its result is taken). */
+int sed_server_submit(sed_server* srv, sed_sector* sec, int nev, double tol, uint64_t seed, int maxiter,
+                      int precision, int64_t* request_id);
+/* Drive the service from the caller's thread: admit queued requests that fit, then
+ * wait: max_steps < 0 returns at once, max_steps == 0 waits until every request
+ * submitted so far has finished, max_steps > 0 waits until that many more have
+ * finished.  Admitted requests keep running (and queued ones keep being admitted)
+ * between calls.  n_done receives the number of requests finished so far. */
+int sed_server_run(sed_server* srv, int max_steps, int64_t* n_done);
+/* Result of a request: status 0 = done, 1 = pending, 2 = running, <0 = failed (error
+ * code); evals (nev doubles) and stats are filled when done. */
+int sed_server_result(sed_server* srv, int64_t request_id, int* status, double* evals, sed_solve_stats* stats,
+                      double* latency_s);
+int sed_server_destroy(sed_server* srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SED_H */

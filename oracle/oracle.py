@@ -1,0 +1,278 @@
+"""ctypes front end of the CPU oracle (``oracle/sed_oracle.c``).
+
+TEST INFRASTRUCTURE: only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` use this module.  It checks the HIP path;
+it is never part of it.
+
+Each function mirrors the reference function of the same name
+(``src/symmetry_resolved_ed.jl``); eigenvalues come from LAPACK (dense) or
+ARPACK via scipy (sparse, the eigensolver of the reference's matrix mode,
+``:978-995``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libsed_oracle.so")
+_lib = None
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.oc_binomial.restype = C.c_int64
+        L.oc_binomial.argtypes = [C.c_int, C.c_int]
+        L.oc_rank_combination.restype = C.c_int64
+        L.oc_rank_combination.argtypes = [C.c_uint64]
+        L.oc_gosper_next.restype = C.c_uint64
+        L.oc_gosper_next.argtypes = [C.c_uint64]
+        L.oc_lin_table.restype = C.c_int
+        L.oc_lin_table.argtypes = [C.c_int, C.c_int, _i64p, _i64p]
+        L.oc_lin_index.restype = C.c_int64
+        L.oc_lin_index.argtypes = [C.c_uint64, _i64p, _i64p, C.c_int]
+        L.oc_inverse_indices.restype = C.c_int
+        L.oc_inverse_indices.argtypes = [C.c_int, C.c_int, _i32p, _f64p, _i32p]
+        L.oc_catalog_build.restype = C.c_void_p
+        L.oc_catalog_build.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _f64p, C.c_int]
+        L.oc_catalog_free.argtypes = [C.c_void_p]
+        L.oc_catalog_n_orbits.restype = C.c_int64
+        L.oc_catalog_n_orbits.argtypes = [C.c_void_p]
+        L.oc_catalog_n_total.restype = C.c_int64
+        L.oc_catalog_n_total.argtypes = [C.c_void_p]
+        L.oc_catalog_tables.argtypes = [C.c_void_p, _i32p, _i32p, _f64p]
+        L.oc_catalog_orbits.argtypes = [C.c_void_p, _u64p, _i64p, _i32p, _f64p]
+        L.oc_canonical_representative.restype = C.c_uint64
+        L.oc_canonical_representative.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32), _f64p]
+        L.oc_apply_operation.restype = C.c_uint64
+        L.oc_apply_operation.argtypes = [C.c_void_p, C.c_uint64, C.c_int, _f64p]
+        L.oc_sector_build.restype = C.c_void_p
+        L.oc_sector_build.argtypes = [C.c_void_p, _f64p, C.c_double]
+        L.oc_sector_free.argtypes = [C.c_void_p]
+        L.oc_sector_dim.restype = C.c_int64
+        L.oc_sector_dim.argtypes = [C.c_void_p]
+        L.oc_sector_basis.argtypes = [C.c_void_p, _u64p, _i64p]
+        L.oc_project_to_sector.restype = C.c_int64
+        L.oc_project_to_sector.argtypes = [C.c_void_p, C.c_uint64, _f64p]
+        L.oc_project_to_unnormalized_sector.restype = C.c_int64
+        L.oc_project_to_unnormalized_sector.argtypes = [C.c_void_p, C.c_uint64, _f64p]
+        L.oc_table_build.restype = C.c_int64
+        L.oc_table_build.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _i32p, _f64p]
+        L.oc_table_copy.argtypes = [C.c_void_p, _i64p, _i64p, _f64p, _f64p]
+        L.oc_apply_table.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int]
+        L.oc_apply_hamiltonian.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _i32p,
+                                           _f64p, _f64p, _f64p, C.c_int, C.c_int64, C.c_int64]
+        _lib = L
+    return _lib
+
+
+def _c128(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.complex128)).view(np.float64)
+
+
+def binomial(n: int, k: int) -> int:
+    return int(lib().oc_binomial(n, k))
+
+
+def rank_combination(m: int) -> int:
+    return int(lib().oc_rank_combination(int(m)))
+
+
+def gosper_next(x: int) -> int:
+    return int(lib().oc_gosper_next(int(x)))
+
+
+def lin_table(n_site: int, n_filled: int):
+    n_right = n_site // 2
+    left = np.zeros(1 << (n_site - n_right), dtype=np.int64)
+    right = np.zeros(1 << n_right, dtype=np.int64)
+    nr = lib().oc_lin_table(n_site, n_filled, left, right)
+    return left, right, nr
+
+
+def lin_index(m: int, left, right, n_right: int) -> int:
+    return int(lib().oc_lin_index(int(m), left, right, n_right))
+
+
+def inverse_operation_indices(group) -> np.ndarray:
+    perm = np.ascontiguousarray(group.perm, dtype=np.int32)
+    out = np.zeros(group.order, dtype=np.int32)
+    if lib().oc_inverse_indices(group.n_site, group.order, perm, _c128(group.phases), out) != 0:
+        raise RuntimeError("Symmetry operations are not closed under inversion.")
+    return out
+
+
+class Catalog:
+    """``build_symmetry_orbit_catalog`` (``src/symmetry_resolved_ed.jl:371-487``)."""
+
+    def __init__(self, n_site: int, n_filled: int, group, fermionic: bool):
+        self.group = group
+        self.n_site = n_site
+        self.n_filled = n_filled
+        self.fermionic = bool(fermionic)
+        perm = np.ascontiguousarray(group.perm, dtype=np.int32)
+        self._h = lib().oc_catalog_build(n_site, n_filled, group.order, perm, _c128(group.phases),
+                                         1 if fermionic else 0)
+        if not self._h:
+            raise RuntimeError("oracle catalog build failed")
+        L = lib()
+        self.n_orbits = int(L.oc_catalog_n_orbits(self._h))
+        self.n_total = int(L.oc_catalog_n_total(self._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oc_catalog_free(self._h)
+            self._h = None
+
+    def tables(self):
+        """(rep_rank_table int32 1-based, rep_sym_table int32 1-based, rep_amp_table c128)."""
+        rr = np.zeros(self.n_total, dtype=np.int32)
+        rs = np.zeros(self.n_total, dtype=np.int32)
+        ra = np.zeros(self.n_total, dtype=np.complex128)
+        lib().oc_catalog_tables(self._h, rr, rs, ra.view(np.float64))
+        return rr, rs, ra
+
+    def orbits(self):
+        """(reps u64, stab_order i64, stab_gidx (n_orbits, |G|) 1-based 0-padded, stab_phase)."""
+        nG = self.group.order
+        reps = np.zeros(self.n_orbits, dtype=np.uint64)
+        so = np.zeros(self.n_orbits, dtype=np.int64)
+        gi = np.zeros((self.n_orbits, nG), dtype=np.int32)
+        gp = np.zeros((self.n_orbits, nG), dtype=np.complex128)
+        lib().oc_catalog_orbits(self._h, reps, so, gi.reshape(-1), gp.view(np.float64).reshape(-1))
+        return reps, so, gi, gp
+
+    def canonical_representative(self, m: int):
+        g = C.c_int32(0)
+        a = np.zeros(2)
+        r = lib().oc_canonical_representative(self._h, int(m), C.byref(g), a)
+        return int(r), int(g.value), complex(a[0], a[1])
+
+    def apply_operation(self, m: int, g1: int):
+        a = np.zeros(2)
+        r = lib().oc_apply_operation(self._h, int(m), int(g1), a)
+        return int(r), complex(a[0], a[1])
+
+
+class Sector:
+    """``build_symmetry_sector_basis`` + projection table + matvecs."""
+
+    def __init__(self, catalog: Catalog, chi, atol: float = 1e-12):
+        self.catalog = catalog
+        self.chi = np.asarray(chi, dtype=np.complex128)
+        self._h = lib().oc_sector_build(catalog._h, _c128(self.chi), atol)
+        self.dim = int(lib().oc_sector_dim(self._h))
+        self.nnz = None
+        self._terms = None
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oc_sector_free(self._h)
+            self._h = None
+
+    def basis(self):
+        reps = np.zeros(self.dim, dtype=np.uint64)
+        stab = np.zeros(self.dim, dtype=np.int64)
+        lib().oc_sector_basis(self._h, reps, stab)
+        return reps, stab
+
+    def project_to_sector(self, m: int):
+        c = np.zeros(2)
+        r = lib().oc_project_to_sector(self._h, int(m), c)
+        return None if r == 0 else (int(r), complex(c[0], c[1]))
+
+    def project_to_unnormalized_sector(self, m: int):
+        c = np.zeros(2)
+        r = lib().oc_project_to_unnormalized_sector(self._h, int(m), c)
+        return None if r == 0 else (int(r), complex(c[0], c[1]))
+
+    def set_terms(self, i_from, i_to, t, di, dj, V):
+        self._terms = (np.ascontiguousarray(i_from, dtype=np.int32), np.ascontiguousarray(i_to, dtype=np.int32),
+                       _c128(t), np.ascontiguousarray(di, dtype=np.int32), np.ascontiguousarray(dj, dtype=np.int32),
+                       _c128(V))
+        a, b, tt, c, d, vv = self._terms
+        self.nnz = int(lib().oc_table_build(self._h, len(a), a, b, tt, len(c), c, d, vv))
+        return self.nnz
+
+    def table(self):
+        """``build_matrixfree_projection_table`` output: (row, col) 1-based, vals, h_diag."""
+        row = np.zeros(self.nnz, dtype=np.int64)
+        col = np.zeros(self.nnz, dtype=np.int64)
+        vals = np.zeros(self.nnz, dtype=np.complex128)
+        hd = np.zeros(self.dim, dtype=np.complex128)
+        lib().oc_table_copy(self._h, row, col, vals.view(np.float64), hd.view(np.float64))
+        return row, col, vals, hd
+
+    def apply_table(self, x, nthreads: int = 1) -> np.ndarray:
+        """``_apply_table!`` with ``nthreads`` per-thread buffers."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros(self.dim, dtype=np.complex128)
+        lib().oc_apply_table(self._h, x.view(np.float64), y.view(np.float64), nthreads)
+        return y
+
+    def apply_hamiltonian(self, x, nthreads: int = 1, c0: int = 0, c1: Optional[int] = None) -> np.ndarray:
+        """On-the-fly ``apply_hamiltonian!`` over columns [c0, c1)."""
+        a, b, tt, c, d, vv = self._terms
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros(self.dim, dtype=np.complex128)
+        lib().oc_apply_hamiltonian(self._h, len(a), a, b, tt, len(c), c, d, vv, x.view(np.float64),
+                                   y.view(np.float64), nthreads, c0, self.dim if c1 is None else c1)
+        return y
+
+    def sparse(self):
+        """The sector block as a scipy CSR matrix (duplicates summed, as ``sparse()`` does)."""
+        import scipy.sparse as sp
+        row, col, vals, hd = self.table()
+        n = self.dim
+        r = np.concatenate([row - 1, np.arange(n)])
+        c = np.concatenate([col - 1, np.arange(n)])
+        v = np.concatenate([vals, hd])
+        return sp.csr_matrix((v, (r, c)), shape=(n, n))
+
+    def dense(self) -> np.ndarray:
+        return self.sparse().toarray()
+
+
+def sector_for(model, group, chi) -> Sector:
+    """Catalog + sector + projection table for a model (lattice builder output)."""
+    from realspace_exactdiagonalization_amd.lattice import term_arrays, FERMIONIC
+    cat = Catalog(model.n_site, model.n_filled, group, model.statistics == FERMIONIC)
+    sec = Sector(cat, chi)
+    sec.set_terms(*term_arrays(model))
+    return sec
+
+
+def lowest_eigenvalues(sec: Sector, nev: int, tol: float = 1e-13) -> np.ndarray:
+    """Lowest ``nev`` eigenvalues: dense LAPACK for small blocks, ARPACK (scipy) otherwise."""
+    n = sec.dim
+    if n == 0:
+        return np.zeros(0)
+    if n <= 2000:
+        H = sec.dense()
+        return np.linalg.eigvalsh(0.5 * (H + H.conj().T))[:nev]
+    from scipy.sparse.linalg import eigsh
+    H = sec.sparse()
+    rng = np.random.default_rng(1234)
+    v0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    vals = eigsh(H, k=nev, which="SA", tol=tol, v0=v0, return_eigenvectors=False, ncv=max(2 * nev + 20, 40))
+    return np.sort(vals.real)

@@ -1,0 +1,622 @@
+// C ABI of libsed: handles, validation, host-side preparation of the device tables.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <utility>
+
+#include "sed_internal.hpp"
+
+namespace sed {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int launch_gauge_rows(sed_sector* sec, const double* f_dev);
+template <typename R>
+size_t matvec_lds_bytes(const sed_sector* sec);
+int launch_density(sed_sector* sec, const double2* vec, double* density);
+int launch_convert(const double* a, float* b, int64_t n, hipStream_t s);
+int launch_convert(const float* a, double* b, int64_t n, hipStream_t s);
+int launch_matvec_kernel_only(sed_sector* sec, const void* xhat, const void* x, void* y, bool fp64, hipStream_t s);
+
+template <typename T>
+static int upload(DevBuf<T>& b, const T* h, size_t n, hipStream_t s) {
+    SED_TRY(b.alloc(n));
+    if (n) SED_HIP(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return SED_OK;
+}
+
+static int max_lds(const sed_context* ctx) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, ctx->device) != hipSuccess) return 65536;
+    return (int)p.sharedMemPerBlock;
+}
+
+}  // namespace sed
+
+using namespace sed;
+
+extern "C" {
+
+const char* sed_last_error(void) { return g_last_error.c_str(); }
+
+int sed_version(void) { return 100; }
+
+int sed_create(int device, sed_context** out) {
+    if (!out) return fail(SED_EINVAL, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    SED_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(SED_EINVAL, "device " + std::to_string(device) + " out of range");
+    SED_HIP(hipSetDevice(device));
+    hipDeviceProp_t p;
+    SED_HIP(hipGetDeviceProperties(&p, device));
+    auto* ctx = new sed_context();
+    ctx->device = device;
+    ctx->n_cu = p.multiProcessorCount;
+    ctx->arch = p.gcnArchName;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(SED_EDEVICE, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+    }
+    ctx->ws = new Workspace();
+    ctx->ws->stream = ctx->stream;
+    *out = ctx;
+    return SED_OK;
+}
+
+int sed_destroy(sed_context* ctx) {
+    if (!ctx) return SED_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->ws) {
+        ctx->ws->release();
+        delete ctx->ws;
+    }
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SED_OK;
+}
+
+int sed_catalog_build(sed_context* ctx, int n_site, int n_filled, int statistics, int n_group, int identity_idx,
+                      const int32_t* perm, const double* phases, sed_catalog** out) {
+    if (!ctx || !perm || !out) return fail(SED_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (n_site < 1 || n_site > 64) return fail(SED_EUNSUPPORTED, "n_site must be in 1..64");
+    if (n_filled < 0 || n_filled > n_site) return fail(SED_EINVAL, "n_filled out of range");
+    if (statistics != SED_BOSONIC && statistics != SED_FERMIONIC) return fail(SED_EINVAL, "bad statistics");
+    if (n_group < 1 || n_group > 64) return fail(SED_EUNSUPPORTED, "group order must be in 1..64");
+    if (identity_idx < 1 || identity_idx > n_group) return fail(SED_EINVAL, "identity_idx out of range");
+    SED_HIP(hipSetDevice(ctx->device));
+    auto* cat = new sed_catalog();
+    std::unique_ptr<sed_catalog> guard(cat);
+    cat->ctx = ctx;
+    cat->n_site = n_site;
+    cat->n_filled = n_filled;
+    cat->stats = statistics;
+    cat->n_group = n_group;
+    cat->identity_idx = identity_idx;
+    cat->perm.assign(perm, perm + (size_t)n_group * n_site);
+    cat->phases.resize((size_t)n_group * n_site);
+    for (size_t i = 0; i < cat->phases.size(); ++i)
+        cat->phases[i] = phases ? cd(phases[2 * i], phases[2 * i + 1]) : cd(1.0, 0.0);
+    // permutations: 1-based, bijective
+    for (int g = 0; g < n_group; ++g) {
+        std::vector<char> seen(n_site, 0);
+        for (int i = 0; i < n_site; ++i) {
+            const int t = cat->perm[(size_t)g * n_site + i];
+            if (t < 1 || t > n_site || seen[t - 1])
+                return fail(SED_EINVAL, "perm of group element " + std::to_string(g + 1) + " is not a permutation");
+            seen[t - 1] = 1;
+        }
+    }
+    for (int i = 0; i < n_site; ++i)
+        if (cat->perm[(size_t)(identity_idx - 1) * n_site + i] != i + 1)
+            return fail(SED_EINVAL, "Identity must have trivial permutation");
+    // _inverse_operation_indices (:109-117): first h with h o g = id and matching phases
+    cat->inv.assign(n_group, -1);
+    for (int g = 0; g < n_group; ++g) {
+        for (int h = 0; h < n_group && cat->inv[g] < 0; ++h) {
+            bool ok = true;
+            for (int i = 0; i < n_site && ok; ++i) {
+                const int gi = cat->perm[(size_t)g * n_site + i];
+                if (cat->perm[(size_t)h * n_site + gi - 1] != i + 1) ok = false;
+            }
+            for (int i = 0; i < n_site && ok; ++i) {
+                const int gi = cat->perm[(size_t)g * n_site + i];
+                if (std::abs(cat->phases[(size_t)g * n_site + i] * cat->phases[(size_t)h * n_site + gi - 1] - 1.0) > 1e-12)
+                    ok = false;
+            }
+            if (ok) cat->inv[g] = h;
+        }
+        if (cat->inv[g] < 0) return fail(SED_EINVAL, "Symmetry operations are not closed under inversion.");
+    }
+    // gauge form of the site phases: eta_g(i) = f(i) / f(perm_g(i))
+    cat->gauge_f.assign(n_site, cd(0.0));
+    std::vector<char> have(n_site, 0);
+    cat->has_phases = false;
+    for (const cd& e : cat->phases)
+        if (e != cd(1.0, 0.0)) cat->has_phases = true;
+    for (int s0 = 0; s0 < n_site; ++s0) {
+        if (have[s0]) continue;
+        have[s0] = 1;
+        cat->gauge_f[s0] = 1.0;
+        std::vector<int> stack{s0};
+        while (!stack.empty()) {
+            const int s = stack.back();
+            stack.pop_back();
+            for (int g = 0; g < n_group; ++g) {
+                const int t = cat->perm[(size_t)g * n_site + s] - 1;
+                if (!have[t]) {
+                    have[t] = 1;
+                    cat->gauge_f[t] = cat->gauge_f[s] / cat->phases[(size_t)g * n_site + s];
+                    stack.push_back(t);
+                }
+            }
+        }
+    }
+    for (int g = 0; g < n_group; ++g)
+        for (int i = 0; i < n_site; ++i) {
+            const int t = cat->perm[(size_t)g * n_site + i] - 1;
+            if (std::abs(cat->phases[(size_t)g * n_site + i] * cat->gauge_f[t] - cat->gauge_f[i]) > 1e-10)
+                return fail(SED_EUNSUPPORTED, "site phases are not of gauge form f(i)/f(perm(i))");
+        }
+    // binomials C(n, j), n <= n_site, j <= n_filled + 1 (saturating at 2^32-1)
+    uint64_t tot = 1;
+    {
+        std::vector<std::vector<uint64_t>> C(n_site + 1, std::vector<uint64_t>(n_site + 2, 0));
+        for (int a = 0; a <= n_site; ++a) {
+            C[a][0] = 1;
+            for (int b = 1; b <= a; ++b) {
+                const uint64_t v = C[a - 1][b - 1] + (b <= a - 1 ? C[a - 1][b] : 0);
+                C[a][b] = std::min<uint64_t>(v, (uint64_t)1 << 62);
+            }
+        }
+        tot = C[n_site][n_filled];
+        if (tot >= (1ull << 31))
+            return fail(SED_EUNSUPPORTED, "C(n_site, n_filled) = " + std::to_string(tot) + " exceeds 2^31");
+        cat->bstride = n_filled + 2;
+        cat->binom.assign((size_t)(n_site + 1) * cat->bstride, 0);
+        for (int a = 0; a <= n_site; ++a)
+            for (int b = 0; b < cat->bstride && b <= n_site + 1; ++b)
+                cat->binom[(size_t)a * cat->bstride + b] = (uint32_t)std::min<uint64_t>(C[a][b], 0xffffffffull);
+    }
+    cat->n_total = (int64_t)tot;
+    hipStream_t s = ctx->stream;
+    std::vector<uint8_t> p8((size_t)n_group * n_site), inv8(n_group);
+    for (size_t i = 0; i < p8.size(); ++i) p8[i] = (uint8_t)(cat->perm[i] - 1);
+    for (int g = 0; g < n_group; ++g) inv8[g] = (uint8_t)cat->inv[g];
+    SED_TRY(upload(cat->d_binom, cat->binom.data(), cat->binom.size(), s));
+    SED_TRY(upload(cat->d_perm, p8.data(), p8.size(), s));
+    SED_TRY(upload(cat->d_inv, inv8.data(), inv8.size(), s));
+    SED_TRY(launch_catalog(cat));
+    *out = guard.release();
+    return SED_OK;
+}
+
+int sed_catalog_info(const sed_catalog* cat, int64_t* n_total, int64_t* n_orbits) {
+    if (!cat) return fail(SED_EINVAL, "NULL catalog");
+    if (n_total) *n_total = cat->n_total;
+    if (n_orbits) *n_orbits = cat->n_orbits;
+    return SED_OK;
+}
+
+int sed_catalog_copy(const sed_catalog* cat, int32_t* rep_rank, int32_t* rep_sym, double* rep_amp, uint64_t* reps,
+                     int64_t* stab_order) {
+    if (!cat) return fail(SED_EINVAL, "NULL catalog");
+    SED_HIP(hipSetDevice(cat->ctx->device));
+    hipStream_t s = cat->ctx->stream;
+    const int64_t n = cat->n_total, no = cat->n_orbits;
+    if (rep_rank) {
+        SED_HIP(hipMemcpyAsync(rep_rank, cat->d_rep_rank.p, n * 4, hipMemcpyDeviceToHost, s));
+        SED_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; ++i) rep_rank[i] = -rep_rank[i];  // -(orbit+1) -> orbit+1
+    }
+    if (rep_sym) {
+        std::vector<uint8_t> sp(n);
+        SED_HIP(hipMemcpyAsync(sp.data(), cat->d_sym_par.p, n, hipMemcpyDeviceToHost, s));
+        SED_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; ++i) rep_sym[i] = (sp[i] & 0x7f) + 1;
+    }
+    if (rep_amp) {
+        DevBuf<double> amp;
+        SED_TRY(amp.alloc((size_t)2 * n));
+        SED_TRY(launch_catalog_amp(cat, amp.p));
+        SED_HIP(hipMemcpy(rep_amp, amp.p, (size_t)16 * n, hipMemcpyDeviceToHost));
+    }
+    if (reps) {
+        SED_HIP(hipMemcpyAsync(reps, cat->d_orbit_rep.p, no * 8, hipMemcpyDeviceToHost, s));
+        SED_HIP(hipStreamSynchronize(s));
+    }
+    if (stab_order) {
+        std::vector<uint64_t> st(no);
+        SED_HIP(hipMemcpyAsync(st.data(), cat->d_orbit_stab.p, no * 8, hipMemcpyDeviceToHost, s));
+        SED_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < no; ++i) stab_order[i] = __builtin_popcountll(st[i]);
+    }
+    return SED_OK;
+}
+
+int sed_catalog_stabilizers(const sed_catalog* cat, int32_t* stab_gidx, double* stab_phase) {
+    if (!cat) return fail(SED_EINVAL, "NULL catalog");
+    SED_HIP(hipSetDevice(cat->ctx->device));
+    hipStream_t s = cat->ctx->stream;
+    const int64_t no = cat->n_orbits;
+    const int nG = cat->n_group, N = cat->n_site;
+    std::vector<uint64_t> st(no), reps(no);
+    SED_HIP(hipMemcpyAsync(st.data(), cat->d_orbit_stab.p, no * 8, hipMemcpyDeviceToHost, s));
+    SED_HIP(hipMemcpyAsync(reps.data(), cat->d_orbit_rep.p, no * 8, hipMemcpyDeviceToHost, s));
+    SED_HIP(hipStreamSynchronize(s));
+    for (int64_t o = 0; o < no; ++o) {
+        int j = 0;
+        for (int h = 0; h < nG; ++h) {
+            if (!((st[o] >> h) & 1)) continue;
+            // alpha_h(rep): reference product of site phases and reordering sign
+            uint64_t m = reps[o], img = 0;
+            cd a(1.0, 0.0);
+            int par = 1;
+            while (m) {
+                const int i = __builtin_ctzll(m);
+                m &= m - 1;
+                const int p = cat->perm[(size_t)h * N + i];  // 1-based
+                if (cat->stats == SED_FERMIONIC) {
+                    const uint64_t above = p >= 64 ? 0 : (img >> p);
+                    if (__builtin_popcountll(above) & 1) par = -par;
+                }
+                img |= 1ull << (p - 1);
+                const cd b = cat->phases[(size_t)h * N + i];
+                a = cd(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+            }
+            a *= (double)par;
+            if (stab_gidx) stab_gidx[o * nG + j] = h + 1;
+            if (stab_phase) {
+                stab_phase[2 * (o * nG + j)] = a.real();
+                stab_phase[2 * (o * nG + j) + 1] = a.imag();
+            }
+            ++j;
+        }
+        for (; j < nG; ++j) {
+            if (stab_gidx) stab_gidx[o * nG + j] = 0;
+            if (stab_phase) stab_phase[2 * (o * nG + j)] = stab_phase[2 * (o * nG + j) + 1] = 0.0;
+        }
+    }
+    return SED_OK;
+}
+
+int sed_catalog_destroy(sed_catalog* cat) {
+    if (!cat) return SED_OK;
+    (void)hipSetDevice(cat->ctx->device);
+    cat->d_binom.release();
+    cat->d_perm.release();
+    cat->d_inv.release();
+    cat->d_rep_rank.release();
+    cat->d_sym_par.release();
+    cat->d_orbit_rep.release();
+    cat->d_orbit_stab.release();
+    cat->d_orbit_stab_par.release();
+    delete cat;
+    return SED_OK;
+}
+
+int sed_sector_build(sed_catalog* cat, const double* chi, double atol, sed_sector** out, int64_t* dim) {
+    if (!cat || !chi || !out) return fail(SED_EINVAL, "NULL argument");
+    *out = nullptr;
+    SED_HIP(hipSetDevice(cat->ctx->device));
+    auto* sec = new sed_sector();
+    std::unique_ptr<sed_sector> guard(sec);
+    sec->cat = cat;
+    sec->atol = atol;
+    sec->chi.resize(cat->n_group);
+    for (int g = 0; g < cat->n_group; ++g) sec->chi[g] = cd(chi[2 * g], chi[2 * g + 1]);
+    hipStream_t s = cat->ctx->stream;
+    SED_TRY(upload(sec->d_chi64, chi, (size_t)2 * cat->n_group, s));
+    std::vector<float> c32(2 * cat->n_group);
+    for (int i = 0; i < 2 * cat->n_group; ++i) c32[i] = (float)chi[i];
+    SED_TRY(upload(sec->d_chi32, c32.data(), c32.size(), s));
+    SED_TRY(launch_sector(sec, sec->d_chi64.p));
+    sec->has_gauge = cat->has_phases;
+    if (sec->has_gauge && sec->dim > 0) {
+        std::vector<double> f(2 * cat->n_site);
+        for (int i = 0; i < cat->n_site; ++i) {
+            f[2 * i] = cat->gauge_f[i].real();
+            f[2 * i + 1] = cat->gauge_f[i].imag();
+        }
+        DevBuf<double> fd;
+        SED_TRY(upload(fd, f.data(), f.size(), s));
+        SED_TRY(sec->d_gauge.alloc((size_t)2 * sec->dim));
+        SED_TRY(launch_gauge_rows(sec, fd.p));
+        SED_HIP(hipStreamSynchronize(s));
+    }
+    if (dim) *dim = sec->dim;
+    ++cat->n_sectors_alive;
+    *out = guard.release();
+    return SED_OK;
+}
+
+int sed_sector_basis(const sed_sector* sec, uint64_t* reps, int64_t* stab_order) {
+    if (!sec) return fail(SED_EINVAL, "NULL sector");
+    SED_HIP(hipSetDevice(sec->cat->ctx->device));
+    hipStream_t s = sec->cat->ctx->stream;
+    if (reps) SED_HIP(hipMemcpyAsync(reps, sec->d_rep.p, sec->dim * 8, hipMemcpyDeviceToHost, s));
+    if (stab_order) {
+        std::vector<uint8_t> st(sec->dim);
+        SED_HIP(hipMemcpyAsync(st.data(), sec->d_stab.p, sec->dim, hipMemcpyDeviceToHost, s));
+        SED_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < sec->dim; ++i) stab_order[i] = st[i];
+    }
+    SED_HIP(hipStreamSynchronize(s));
+    return SED_OK;
+}
+
+int sed_sector_set_terms(sed_sector* sec, int n_bil, const int32_t* i_from, const int32_t* i_to, const double* t,
+                         int n_den, const int32_t* di, const int32_t* dj, const double* v) {
+    if (!sec) return fail(SED_EINVAL, "NULL sector");
+    if (n_bil < 0 || n_den < 0) return fail(SED_EINVAL, "negative term count");
+    if ((n_bil && (!i_from || !i_to || !t)) || (n_den && (!di || !dj || !v)))
+        return fail(SED_EINVAL, "NULL term array");
+    const sed_catalog* cat = sec->cat;
+    const int N = cat->n_site;
+    SED_HIP(hipSetDevice(cat->ctx->device));
+    // merge hops with equal (from, to): their matrix contributions add
+    std::map<std::pair<int, int>, cd> hops;
+    for (int k = 0; k < n_bil; ++k) {
+        const int a = i_from[k], b = i_to[k];
+        if (a < 1 || a > N || b < 1 || b > N) return fail(SED_EINVAL, "bilinear term site out of range");
+        if (a == b) return fail(SED_EINVAL, "bilinear term with i_from == i_to changes the particle number");
+        // gauge-transformed amplitude t' = t f(to)/f(from); the kernel uses conj(t')
+        const cd tp = cd(t[2 * k], t[2 * k + 1]) * cat->gauge_f[b - 1] / cat->gauge_f[a - 1];
+        hops[{a - 1, b - 1}] += tp;
+    }
+    std::map<std::pair<int, int>, cd> dens;  // ordered by first site
+    std::vector<std::vector<std::pair<int, cd>>> dl(N);
+    for (int k = 0; k < n_den; ++k) {
+        const int a = di[k], b = dj[k];
+        if (a < 1 || a > N || b < 1 || b > N) return fail(SED_EINVAL, "density term site out of range");
+        dl[a - 1].push_back({b - 1, cd(v[2 * k], v[2 * k + 1])});
+    }
+    sec->term_start.assign(N + 1, 0);
+    sec->term_to.clear();
+    sec->term_coef.clear();
+    for (int a = 0; a < N; ++a) {
+        sec->term_start[a] = (int)sec->term_to.size();
+        for (auto it = hops.lower_bound({a, -1}); it != hops.end() && it->first.first == a; ++it) {
+            sec->term_to.push_back((uint8_t)it->first.second);
+            sec->term_coef.push_back(std::conj(it->second));
+        }
+    }
+    sec->term_start[N] = (int)sec->term_to.size();
+    sec->dens_start.assign(N + 1, 0);
+    sec->dens_j.clear();
+    sec->dens_v.clear();
+    for (int a = 0; a < N; ++a) {
+        sec->dens_start[a] = (int)sec->dens_j.size();
+        for (auto& pr : dl[a]) {
+            sec->dens_j.push_back((uint8_t)pr.first);
+            sec->dens_v.push_back(pr.second);
+        }
+    }
+    sec->dens_start[N] = (int)sec->dens_j.size();
+    sec->n_terms = (int)sec->term_to.size();
+    sec->n_dens = (int)sec->dens_j.size();
+    hipStream_t s = cat->ctx->stream;
+    std::vector<double> tc64(2 * std::max(1, sec->n_terms)), dv64(2 * std::max(1, sec->n_dens));
+    std::vector<float> tc32(tc64.size()), dv32(dv64.size());
+    for (int i = 0; i < sec->n_terms; ++i) {
+        tc64[2 * i] = sec->term_coef[i].real();
+        tc64[2 * i + 1] = sec->term_coef[i].imag();
+    }
+    for (int i = 0; i < sec->n_dens; ++i) {
+        dv64[2 * i] = sec->dens_v[i].real();
+        dv64[2 * i + 1] = sec->dens_v[i].imag();
+    }
+    for (size_t i = 0; i < tc64.size(); ++i) tc32[i] = (float)tc64[i];
+    for (size_t i = 0; i < dv64.size(); ++i) dv32[i] = (float)dv64[i];
+    std::vector<uint8_t> tto = sec->term_to, djv = sec->dens_j;
+    if (tto.empty()) tto.push_back(0);
+    if (djv.empty()) djv.push_back(0);
+    SED_TRY(upload(sec->d_term_start, sec->term_start.data(), sec->term_start.size(), s));
+    SED_TRY(upload(sec->d_dens_start, sec->dens_start.data(), sec->dens_start.size(), s));
+    SED_TRY(upload(sec->d_term_to, tto.data(), tto.size(), s));
+    SED_TRY(upload(sec->d_dens_j, djv.data(), djv.size(), s));
+    SED_TRY(upload(sec->d_term_coef64, tc64.data(), tc64.size(), s));
+    SED_TRY(upload(sec->d_dens_v64, dv64.data(), dv64.size(), s));
+    SED_TRY(upload(sec->d_term_coef32, tc32.data(), tc32.size(), s));
+    SED_TRY(upload(sec->d_dens_v32, dv32.data(), dv32.size(), s));
+    SED_HIP(hipStreamSynchronize(s));
+    const size_t need = std::max(matvec_lds_bytes<double>(sec), matvec_lds_bytes<float>(sec));
+    if ((int)need > max_lds(cat->ctx))
+        return fail(SED_EUNSUPPORTED, "term lists need " + std::to_string(need) + " bytes of LDS per workgroup");
+    sec->has_terms = true;
+    sec->hops_gated = sec->hops_in_sector = -1;
+    return SED_OK;
+}
+
+int sed_sector_hop_counts(sed_sector* sec, int64_t* hops_gated, int64_t* hops_in_sector) {
+    if (!sec) return fail(SED_EINVAL, "NULL sector");
+    if (!sec->has_terms) return fail(SED_ESTATE, "hop counts before sed_sector_set_terms");
+    SED_HIP(hipSetDevice(sec->cat->ctx->device));
+    if (sec->hops_gated < 0) SED_TRY(launch_hop_counts(sec, &sec->hops_gated, &sec->hops_in_sector));
+    if (hops_gated) *hops_gated = sec->hops_gated;
+    if (hops_in_sector) *hops_in_sector = sec->hops_in_sector;
+    return SED_OK;
+}
+
+// y = H x with x, y complex double vectors on the device, reference basis
+static int matvec_dev64(sed_sector* sec, const double* x, double* y, int precision) {
+    hipStream_t s = sec->cat->ctx->stream;
+    const int64_t n = sec->dim;
+    DevBuf<double> xp;
+    SED_TRY(xp.alloc((size_t)2 * n));
+    SED_HIP(hipMemcpyAsync(xp.p, x, (size_t)16 * n, hipMemcpyDeviceToDevice, s));
+    SED_TRY(launch_gauge64(sec, xp.p, true, s));
+    Workspace& ws = *sec->cat->ctx->ws;
+    if (precision == SED_PREC_FP64) {
+        SED_TRY(grow(ws.xh64, (size_t)2 * n));
+        SED_TRY(launch_matvec64(sec, xp.p, y, ws.xh64.p, s));
+    } else {
+        DevBuf<float> x32, y32;
+        SED_TRY(x32.alloc((size_t)2 * n));
+        SED_TRY(y32.alloc((size_t)2 * n));
+        SED_TRY(grow(ws.xh32, (size_t)2 * n));
+        SED_TRY(launch_convert(xp.p, x32.p, n, s));
+        SED_TRY(launch_matvec32(sec, x32.p, y32.p, ws.xh32.p, s));
+        SED_TRY(launch_convert(y32.p, y, n, s));
+    }
+    SED_TRY(launch_gauge64(sec, y, false, s));
+    SED_HIP(hipStreamSynchronize(s));
+    return SED_OK;
+}
+
+int sed_matvec(sed_sector* sec, const double* x, double* y, int precision) {
+    if (!sec || !x || !y) return fail(SED_EINVAL, "NULL argument");
+    if (precision != SED_PREC_FP64 && precision != SED_PREC_FP32) return fail(SED_EINVAL, "bad precision");
+    if (!sec->has_terms) return fail(SED_ESTATE, "matvec before sed_sector_set_terms");
+    SED_HIP(hipSetDevice(sec->cat->ctx->device));
+    const int64_t n = sec->dim;
+    if (n == 0) return SED_OK;
+    hipStream_t s = sec->cat->ctx->stream;
+    DevBuf<double> dx, dy;
+    SED_TRY(dx.alloc((size_t)2 * n));
+    SED_TRY(dy.alloc((size_t)2 * n));
+    SED_HIP(hipMemcpyAsync(dx.p, x, (size_t)16 * n, hipMemcpyHostToDevice, s));
+    SED_TRY(matvec_dev64(sec, dx.p, dy.p, precision));
+    SED_HIP(hipMemcpyAsync(y, dy.p, (size_t)16 * n, hipMemcpyDeviceToHost, s));
+    SED_HIP(hipStreamSynchronize(s));
+    return SED_OK;
+}
+
+int sed_matvec_device(sed_sector* sec, const void* x_dev, void* y_dev, int precision) {
+    if (!sec || !x_dev || !y_dev) return fail(SED_EINVAL, "NULL argument");
+    if (!sec->has_terms) return fail(SED_ESTATE, "matvec before sed_sector_set_terms");
+    SED_HIP(hipSetDevice(sec->cat->ctx->device));
+    hipStream_t s = sec->cat->ctx->stream;
+    if (sec->has_gauge && precision == SED_PREC_FP64)
+        return matvec_dev64(sec, (const double*)x_dev, (double*)y_dev, precision);
+    if (sec->has_gauge) return fail(SED_EUNSUPPORTED, "device FP32 matvec with twisted phases: use sed_matvec");
+    Workspace& ws = *sec->cat->ctx->ws;
+    if (precision == SED_PREC_FP64) {
+        SED_TRY(grow(ws.xh64, (size_t)2 * sec->dim));
+        return launch_matvec64(sec, (const double*)x_dev, (double*)y_dev, ws.xh64.p, s);
+    }
+    if (precision == SED_PREC_FP32) {
+        SED_TRY(grow(ws.xh32, (size_t)2 * sec->dim));
+        return launch_matvec32(sec, (const float*)x_dev, (float*)y_dev, ws.xh32.p, s);
+    }
+    return fail(SED_EINVAL, "bad precision");
+}
+
+int sed_matvec_benchmark(sed_sector* sec, int precision, int n_calls, double* ms_per_call) {
+    if (!sec || !ms_per_call || n_calls < 1) return fail(SED_EINVAL, "bad argument");
+    if (precision != SED_PREC_FP64 && precision != SED_PREC_FP32) return fail(SED_EINVAL, "bad precision");
+    if (!sec->has_terms) return fail(SED_ESTATE, "matvec before sed_sector_set_terms");
+    SED_HIP(hipSetDevice(sec->cat->ctx->device));
+    hipStream_t s = sec->cat->ctx->stream;
+    const int64_t n = sec->dim;
+    const bool fp64 = precision == SED_PREC_FP64;
+    const size_t es = fp64 ? 16 : 8;
+    DevBuf<uint8_t> x, y, xh;
+    SED_TRY(x.alloc(es * n));
+    SED_TRY(y.alloc(es * n));
+    SED_TRY(xh.alloc(es * n));
+    // x = 1 + i (any dense vector: every hop gathers a live value)
+    {
+        std::vector<double> h((size_t)2 * n, 1.0);
+        DevBuf<double> d;
+        SED_TRY(d.alloc((size_t)2 * n));
+        SED_HIP(hipMemcpyAsync(d.p, h.data(), (size_t)16 * n, hipMemcpyHostToDevice, s));
+        if (fp64) SED_HIP(hipMemcpyAsync(x.p, d.p, (size_t)16 * n, hipMemcpyDeviceToDevice, s));
+        else SED_TRY(launch_convert(d.p, (float*)x.p, n, s));
+        SED_HIP(hipStreamSynchronize(s));
+    }
+    // one full call (prescale + kernel) fills xhat; timed calls run the H|psi> kernel alone
+    if (fp64) SED_TRY(launch_matvec64(sec, (const double*)x.p, (double*)y.p, (double*)xh.p, s));
+    else SED_TRY(launch_matvec32(sec, (const float*)x.p, (float*)y.p, (float*)xh.p, s));
+    for (int i = 0; i < 2; ++i) SED_TRY(launch_matvec_kernel_only(sec, xh.p, x.p, y.p, fp64, s));
+    hipEvent_t e0, e1;
+    SED_HIP(hipEventCreate(&e0));
+    SED_HIP(hipEventCreate(&e1));
+    SED_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < n_calls; ++i) SED_TRY(launch_matvec_kernel_only(sec, xh.p, x.p, y.p, fp64, s));
+    SED_HIP(hipEventRecord(e1, s));
+    SED_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SED_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_call = ms / n_calls;
+    return SED_OK;
+}
+
+int sed_solve(sed_sector* sec, int nev, double tol, uint64_t seed, int maxiter, int precision, double* evals,
+              double* evecs, sed_solve_stats* stats) {
+    if (!sec || !evals) return fail(SED_EINVAL, "NULL argument");
+    if (nev < 1) return fail(SED_EINVAL, "nev must be >= 1");
+    if (!(tol > 0)) return fail(SED_EINVAL, "tol must be > 0");
+    if (precision != SED_PREC_FP64 && precision != SED_PREC_FP32 && precision != SED_PREC_FP32_REFINE)
+        return fail(SED_EINVAL, "bad precision");
+    if (!sec->has_terms) return fail(SED_ESTATE, "solve before sed_sector_set_terms");
+    SED_HIP(hipSetDevice(sec->cat->ctx->device));
+    std::vector<double> ev;
+    DevBuf<double2> vecs;
+    const int rc = solve_sector(sec, nev, tol, seed, maxiter < 1 ? 300 : maxiter, precision, ev, &vecs, stats,
+                                *sec->cat->ctx->ws);
+    const int64_t n = sec->dim;
+    const int k = (int)ev.size();
+    for (int i = 0; i < k; ++i) evals[i] = ev[i];
+    for (int i = k; i < nev; ++i) evals[i] = NAN;
+    sec->last_evals = ev;
+    sec->last_nev = k;
+    sec->last_evecs.clear();
+    if (k > 0 && n > 0 && vecs.p) {
+        hipStream_t s = sec->cat->ctx->stream;
+        for (int i = 0; i < k; ++i) SED_TRY(launch_gauge64(sec, (double*)(vecs.p + (size_t)i * n), false, s));
+        sec->last_evecs.resize((size_t)n * k);
+        SED_HIP(hipMemcpyAsync(sec->last_evecs.data(), vecs.p, (size_t)16 * n * k, hipMemcpyDeviceToHost, s));
+        SED_HIP(hipStreamSynchronize(s));
+        if (evecs) std::memcpy(evecs, sec->last_evecs.data(), (size_t)16 * n * k);
+    }
+    return rc;
+}
+
+int sed_sector_density(sed_sector* sec, int which, double* density) {
+    if (!sec || !density) return fail(SED_EINVAL, "NULL argument");
+    if (which < 0 || which >= sec->last_nev) return fail(SED_ESTATE, "no eigenvector " + std::to_string(which));
+    SED_HIP(hipSetDevice(sec->cat->ctx->device));
+    DevBuf<double2> v;
+    SED_TRY(v.alloc(sec->dim));
+    SED_HIP(hipMemcpy(v.p, sec->last_evecs.data() + (size_t)which * sec->dim, (size_t)16 * sec->dim,
+                      hipMemcpyHostToDevice));
+    return launch_density(sec, v.p, density);
+}
+
+int sed_sector_destroy(sed_sector* sec) {
+    if (!sec) return SED_OK;
+    (void)hipSetDevice(sec->cat->ctx->device);
+    --sec->cat->n_sectors_alive;
+    for (auto* b : {&sec->d_rep}) b->release();
+    sec->d_stab.release();
+    sec->d_table.release();
+    sec->d_term_start.release();
+    sec->d_dens_start.release();
+    sec->d_term_to.release();
+    sec->d_dens_j.release();
+    sec->d_term_coef64.release();
+    sec->d_dens_v64.release();
+    sec->d_chi64.release();
+    sec->d_term_coef32.release();
+    sec->d_dens_v32.release();
+    sec->d_chi32.release();
+    sec->d_gauge.release();
+    delete sec;
+    return SED_OK;
+}
+
+}  // extern "C"

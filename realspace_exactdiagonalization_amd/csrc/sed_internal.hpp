@@ -1,0 +1,167 @@
+// Internal host-side structures and helpers of libsed (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sed.h"
+
+namespace sed {
+
+using cd = std::complex<double>;
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define SED_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return ::sed::fail(SED_EDEVICE, std::string(#call " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define SED_TRY(call)            \
+    do {                         \
+        int rc_ = (call);        \
+        if (rc_ != SED_OK) return rc_; \
+    } while (0)
+
+// RAII-free device buffer: freed explicitly by the owning handle.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(SED_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                        " bytes failed: " + hipGetErrorString(e));
+        }
+        n = count;
+        return SED_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+struct Workspace;
+
+}  // namespace sed
+
+struct sed_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    std::string arch;
+    sed::Workspace* ws = nullptr;     // workspace of the synchronous API (on `stream`)
+};
+
+struct sed_catalog {
+    sed_context* ctx = nullptr;
+    int n_site = 0, n_filled = 0, stats = 0, n_group = 0, identity_idx = 1;
+    int64_t n_total = 0, n_orbits = 0;
+    std::vector<int32_t> perm;        // 1-based, n_group x n_site
+    std::vector<sed::cd> phases;      // n_group x n_site
+    std::vector<int32_t> inv;         // 0-based inverse element index
+    std::vector<sed::cd> gauge_f;     // f(i): eta_g(i) = f(i)/f(perm_g(i))
+    bool has_phases = false;          // any eta != 1
+    std::vector<uint32_t> binom;      // (n_site+1) x bstride, saturating
+    int bstride = 0;
+    // device state
+    sed::DevBuf<uint32_t> d_binom;
+    sed::DevBuf<uint8_t> d_perm;      // 0-based targets, n_group x n_site
+    sed::DevBuf<uint8_t> d_inv;
+    sed::DevBuf<int32_t> d_rep_rank;  // per rank: -(orbit+1)
+    sed::DevBuf<uint8_t> d_sym_par;   // per rank: group index h (bits 0-6), parity (bit 7)
+    sed::DevBuf<uint64_t> d_orbit_rep;
+    sed::DevBuf<uint64_t> d_orbit_stab;      // bit h set if h(rep) == rep
+    sed::DevBuf<uint64_t> d_orbit_stab_par;  // bit h set if that stabilizer element has sign -1
+    int64_t n_sectors_alive = 0;
+};
+
+struct sed_sector {
+    sed_catalog* cat = nullptr;
+    std::vector<sed::cd> chi;
+    double atol = 1e-12;
+    int64_t dim = 0;
+    int shift = 1;                    // table entry = row << shift | g << 1 | parity
+    sed::DevBuf<uint64_t> d_rep;      // sector representatives (ascending)
+    sed::DevBuf<uint8_t> d_stab;      // stabilizer orders
+    sed::DevBuf<uint32_t> d_table;    // per rank: packed (row, g, parity) or 0xffffffff
+    // terms
+    bool has_terms = false;
+    int n_terms = 0, n_dens = 0;
+    std::vector<int32_t> term_start, dens_start;  // CSR by from-site (0-based), n_site+1
+    std::vector<uint8_t> term_to, dens_j;
+    std::vector<sed::cd> term_coef;   // conj(t'_k), gauge-transformed
+    std::vector<sed::cd> dens_v;
+    bool has_gauge = false;           // D = diag(F(rep)) != 1
+    sed::DevBuf<int32_t> d_term_start, d_dens_start;
+    sed::DevBuf<uint8_t> d_term_to, d_dens_j;
+    sed::DevBuf<double> d_term_coef64, d_dens_v64, d_chi64;  // interleaved complex
+    sed::DevBuf<float> d_term_coef32, d_dens_v32, d_chi32;
+    sed::DevBuf<double> d_gauge;      // F(rep_r) per row, complex interleaved (only if has_gauge)
+    int64_t hops_gated = -1, hops_in_sector = -1;
+    int busy = 0;                     // a server request is running on this sector
+    // last solve
+    std::vector<double> last_evals;
+    std::vector<sed::cd> last_evecs;  // dim x nev, reference basis
+    int last_nev = 0;
+};
+
+namespace sed {
+
+// Reusable device/pinned buffers of one solver lane (one stream).  Grown on demand,
+// never shrunk, so repeated solves do not allocate in the hot loop.
+struct Workspace {
+    hipStream_t stream = nullptr;
+    DevBuf<double> V64, w64, xh64;    // complex interleaved doubles
+    DevBuf<float> V32, w32, xh32;     // complex interleaved floats
+    DevBuf<double> part, coef, S, v0;
+    double* hcoef = nullptr;          // pinned host copy of projection coefficients
+    size_t hcoef_n = 0;
+    int ensure_host(size_t n_doubles);
+    void release();
+};
+template <typename T>
+int grow(DevBuf<T>& b, size_t n) {
+    if (b.n >= n) return SED_OK;
+    return b.alloc(n);
+}
+
+// ---- kernel launchers (device code in the .hip files) ----
+int launch_catalog(sed_catalog* cat);
+int launch_catalog_amp(const sed_catalog* cat, double* d_amp);  // reference rep_amp_table on device
+int launch_sector(sed_sector* sec, const double* chi_dev);
+int launch_hop_counts(sed_sector* sec, int64_t* gated, int64_t* in_sector);
+
+// y = H' x in the sector basis (gauge-transformed, see DESIGN); x, y device vectors.
+// prescale: xhat = sqrt(stab) * x.   FP64: double2, FP32: float2.
+int launch_matvec64(sed_sector* sec, const double* x, double* y, double* xhat, hipStream_t s);
+int launch_matvec32(sed_sector* sec, const float* x, float* y, float* xhat, hipStream_t s);
+// gauge D: v <- D^{+-1} v  (forward = multiply by F, backward by conj F)
+int launch_gauge64(sed_sector* sec, double* v, bool forward, hipStream_t s);
+
+int solve_sector(sed_sector* sec, int nev, double tol, uint64_t seed, int maxiter, int precision,
+                 std::vector<double>& evals, DevBuf<double2>* out_vecs, sed_solve_stats* stats, Workspace& ws);
+// device bytes a solve of this sector needs in a workspace
+int64_t solve_footprint(const sed_sector* sec, int nev, int precision);
+
+// ---- host eigensolvers (sed_eigen.cpp) ----
+// Real symmetric n x n (row-major, overwritten) -> ascending eigenvalues, eigenvectors in columns.
+void jacobi_eigh(int n, std::vector<double>& a, std::vector<double>& evals, std::vector<double>& evecs);
+// Complex Hermitian n x n (column-major) -> ascending eigenvalues and eigenvectors (columns).
+void hermitian_eigh(int n, std::vector<cd>& a, std::vector<double>& evals, std::vector<cd>* evecs);
+
+}  // namespace sed

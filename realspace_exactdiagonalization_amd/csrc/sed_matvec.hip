@@ -1,0 +1,510 @@
+// Row-gather H|psi> for one symmetry sector on gfx950.
+//
+// Replaces MatrixFreeHamiltonian / _apply_table! (src/symmetry_resolved_ed.jl:731-791,
+// :880-884) and the on-the-fly apply_hamiltonian! (:813-878).  The reference scatters
+// column contributions H[row, col] x[col] into per-thread buffers; here one thread owns
+// one output row r and gathers, using Hermiticity H[r, c] = conj(H[c, r]):
+//
+//   H[c, r] = t_k * s_JW(m_r) * alpha * conj(chi(g)) * sqrt(stab_c / stab_r)
+//
+// for every term k = (a -> b) with site a occupied and b empty in the representative
+// m_r (the reference's gate (m & hop) == from, :709), where (c, g, alpha) is the
+// canonical image of m_r ^ hop (project_to_sector, :627-632).  No atomics, and every
+// row is summed in a fixed order, so results are bitwise reproducible.
+//
+// Per hop the thread needs the combinadic rank of the new mask.  With m' = m_r minus
+// bit a (bits q_0 < ... < q_{k-2}), inserting b at position i_b = popcount(m' below b)
+// gives  rank = S[i_b] + C(b, i_b + 1)  where
+//   S[i] = sum_{j<i} C(q_j, j+1) + sum_{j>=i} C(q_j, j+2),
+// built once per occupied site a (k values, kept in LDS).  The Jordan-Wigner sign of
+// hopping_phase_for_stats (:646-654) is (-1)^|i_b - i_a| with i_a the index of a in m_r.
+//
+// Table entry (4 bytes per combinadic rank): row << shift | g << 1 | parity, or
+// 0xffffffff when the orbit is not in the sector.  The vector gathered is
+// xhat = sqrt(stab) * x, so the stabilizer norm of the target row costs no load.
+//
+// Twisted (flux) groups: with site phases eta_g(i) = f(i)/f(perm_g(i)), the reference's
+// matrix equals D^-1 H' D with D = diag(prod_{i in rep} f(i)) and H' built from the
+// sign-only tables and terms t'_k = t_k f(b)/f(a).  This kernel applies H'; the host
+// applies D where vectors cross the boundary.
+#include <algorithm>
+#include <cmath>
+
+#include "sed_device.hpp"
+#include "sed_internal.hpp"
+
+namespace sed {
+
+constexpr int kMvBlock = 256;
+
+struct MvConst {
+    const uint32_t* bin;
+    const int32_t* tstart;
+    const uint8_t* tto;
+    const void* tcoef;
+    const int32_t* dstart;
+    const uint8_t* dj;
+    const void* dv;
+    const void* chi;
+    int n, k, nG, bs, shift, n_terms, n_dens;
+    uint32_t gmask;
+    int64_t dim;
+};
+
+template <typename R>
+struct MvLds {
+    using T = typename C2<R>::T;
+    uint32_t* bin;
+    int32_t* tstart;
+    int32_t* dstart;
+    T* tcoef;
+    T* dv;
+    T* chi;
+    uint8_t* tto;
+    uint8_t* dj;
+    uint32_t* S;
+};
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+template <typename R>
+__host__ __device__ inline size_t mv_lds_layout(const MvConst& c, size_t* off) {
+    using T = typename C2<R>::T;
+    size_t o = 0;
+    off[0] = o; o = align16(o + (size_t)(c.n + 1) * c.bs * 4);       // bin
+    off[1] = o; o = align16(o + (size_t)(c.n + 1) * 4);               // tstart
+    off[2] = o; o = align16(o + (size_t)(c.n + 1) * 4);               // dstart
+    off[3] = o; o = align16(o + (size_t)c.n_terms * sizeof(T));       // tcoef
+    off[4] = o; o = align16(o + (size_t)c.n_dens * sizeof(T));        // dv
+    off[5] = o; o = align16(o + (size_t)c.nG * sizeof(T));            // chi
+    off[6] = o; o = align16(o + (size_t)c.n_terms);                   // tto
+    off[7] = o; o = align16(o + (size_t)c.n_dens);                    // dj
+    off[8] = o; o = align16(o + (size_t)(c.k > 0 ? c.k : 1) * kMvBlock * 4);  // S
+    return o;
+}
+
+template <typename R>
+__device__ __forceinline__ MvLds<R> stage_mv(const MvConst& c) {
+    using T = typename C2<R>::T;
+    extern __shared__ __align__(16) uint8_t smem[];
+    size_t off[9];
+    mv_lds_layout<R>(c, off);
+    MvLds<R> L;
+    L.bin = reinterpret_cast<uint32_t*>(smem + off[0]);
+    L.tstart = reinterpret_cast<int32_t*>(smem + off[1]);
+    L.dstart = reinterpret_cast<int32_t*>(smem + off[2]);
+    L.tcoef = reinterpret_cast<T*>(smem + off[3]);
+    L.dv = reinterpret_cast<T*>(smem + off[4]);
+    L.chi = reinterpret_cast<T*>(smem + off[5]);
+    L.tto = smem + off[6];
+    L.dj = smem + off[7];
+    L.S = reinterpret_cast<uint32_t*>(smem + off[8]);
+    const int nt = blockDim.x;
+    for (int i = threadIdx.x; i < (c.n + 1) * c.bs; i += nt) L.bin[i] = c.bin[i];
+    for (int i = threadIdx.x; i <= c.n; i += nt) {
+        L.tstart[i] = c.tstart[i];
+        L.dstart[i] = c.dstart[i];
+    }
+    const T* tc = reinterpret_cast<const T*>(c.tcoef);
+    const T* dv = reinterpret_cast<const T*>(c.dv);
+    const T* ch = reinterpret_cast<const T*>(c.chi);
+    for (int i = threadIdx.x; i < c.n_terms; i += nt) {
+        L.tcoef[i] = tc[i];
+        L.tto[i] = c.tto[i];
+    }
+    for (int i = threadIdx.x; i < c.n_dens; i += nt) {
+        L.dv[i] = dv[i];
+        L.dj[i] = c.dj[i];
+    }
+    for (int i = threadIdx.x; i < c.nG; i += nt) L.chi[i] = ch[i];
+    __syncthreads();
+    return L;
+}
+
+template <typename M, typename R, bool FERMI>
+__global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* __restrict__ rep,
+                                                     const uint8_t* __restrict__ stab,
+                                                     const uint32_t* __restrict__ table,
+                                                     const typename C2<R>::T* __restrict__ xhat,
+                                                     const typename C2<R>::T* __restrict__ x,
+                                                     typename C2<R>::T* __restrict__ y, int64_t nblocks) {
+    using T = typename C2<R>::T;
+    const MvLds<R> L = stage_mv<R>(c);
+    const int64_t lb = xcd_block(nblocks);
+    if (lb < 0) return;
+    const int64_t r = lb * kMvBlock + threadIdx.x;
+    if (r >= c.dim) return;
+    const M m = (M)rep[r];
+    uint32_t* S = L.S + threadIdx.x;
+    R ax = 0, ay = 0;   // off-diagonal sum
+    R dx = 0, dy = 0;   // diagonal (density) sum
+    M rest = m;
+    int ia = 0;
+    while (rest) {
+        const int a = ctz_m<M>(rest);
+        rest &= rest - 1;
+        const M mp = m ^ ((M)1 << a);
+        {
+            uint32_t s0 = 0;
+            M t = mp;
+            int j = 0;
+            while (t) {
+                const int q = ctz_m<M>(t);
+                t &= t - 1;
+                s0 += L.bin[q * c.bs + j + 2];
+                ++j;
+            }
+            S[0] = s0;
+            t = mp;
+            j = 0;
+            uint32_t cur = s0;
+            while (t) {
+                const int q = ctz_m<M>(t);
+                t &= t - 1;
+                cur += L.bin[q * c.bs + j + 1] - L.bin[q * c.bs + j + 2];
+                ++j;
+                S[j * kMvBlock] = cur;
+            }
+        }
+        for (int d = L.dstart[a]; d < L.dstart[a + 1]; ++d) {
+            if ((m >> L.dj[d]) & 1) {
+                dx += L.dv[d].x;
+                dy += L.dv[d].y;
+            }
+        }
+        const int t1 = L.tstart[a + 1];
+        for (int t = L.tstart[a]; t < t1; ++t) {
+            const int b = L.tto[t];
+            if ((m >> b) & 1) continue;
+            const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
+            const uint32_t rank = S[ib * kMvBlock] + L.bin[b * c.bs + ib + 1];
+            const uint32_t e = table[rank];
+            if (e == kNoRow) continue;
+            const uint32_t col = e >> c.shift;
+            const int g = (e >> 1) & c.gmask;
+            int p = e & 1;
+            if (FERMI) p ^= (ib - ia) & 1;
+            const T cf = cmul(L.tcoef[t], L.chi[g]);
+            const T v = xhat[col];
+            const T pr = cmul(cf, v);
+            if (p) {
+                ax -= pr.x;
+                ay -= pr.y;
+            } else {
+                ax += pr.x;
+                ay += pr.y;
+            }
+        }
+        ++ia;
+    }
+    const R inv = (R)1 / sqrt((R)stab[r]);
+    const T xr = x[r];
+    T out;
+    out.x = ax * inv + (dx * xr.x - dy * xr.y);
+    out.y = ay * inv + (dx * xr.y + dy * xr.x);
+    y[r] = out;
+}
+
+// Dense H' (row-major, dim x dim, complex double) for the small-sector path.
+template <typename M, bool FERMI>
+__global__ void __launch_bounds__(kMvBlock) k_dense(MvConst c, const uint64_t* __restrict__ rep,
+                                                    const uint8_t* __restrict__ stab,
+                                                    const uint32_t* __restrict__ table,
+                                                    double2* __restrict__ H) {
+    const MvLds<double> L = stage_mv<double>(c);
+    const int64_t r = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
+    if (r >= c.dim) return;
+    const M m = (M)rep[r];
+    const double inv = 1.0 / sqrt((double)stab[r]);
+    double2* row = H + r * c.dim;
+    M rest = m;
+    int ia = 0;
+    double dx = 0, dy = 0;
+    while (rest) {
+        const int a = ctz_m<M>(rest);
+        rest &= rest - 1;
+        const M mp = m ^ ((M)1 << a);
+        for (int d = L.dstart[a]; d < L.dstart[a + 1]; ++d) {
+            if ((m >> L.dj[d]) & 1) {
+                dx += L.dv[d].x;
+                dy += L.dv[d].y;
+            }
+        }
+        for (int t = L.tstart[a]; t < L.tstart[a + 1]; ++t) {
+            const int b = L.tto[t];
+            if ((m >> b) & 1) continue;
+            const M nm = mp | ((M)1 << b);
+            const uint32_t rank = colex_rank<M>(nm, L.bin, c.bs);
+            const uint32_t e = table[rank];
+            if (e == kNoRow) continue;
+            const uint32_t col = e >> c.shift;
+            const int g = (e >> 1) & c.gmask;
+            int p = e & 1;
+            const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
+            if (FERMI) p ^= (ib - ia) & 1;
+            double2 cf = cmul(L.tcoef[t], L.chi[g]);
+            const double sc = sqrt((double)stab[col]) * inv * (p ? -1.0 : 1.0);
+            row[col].x += cf.x * sc;
+            row[col].y += cf.y * sc;
+        }
+        ++ia;
+    }
+    row[r].x += dx;
+    row[r].y += dy;
+}
+
+// Exact counts of gated hops (one table read each) and in-sector hops (one gather).
+template <typename M>
+__global__ void __launch_bounds__(kMvBlock) k_hop_counts(MvConst c, const uint64_t* __restrict__ rep,
+                                                         const uint32_t* __restrict__ table,
+                                                         unsigned long long* __restrict__ partial) {
+    const MvLds<double> L = stage_mv<double>(c);
+    __shared__ unsigned long long red[2][kMvBlock / kWave];
+    const int64_t r = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
+    unsigned long long ng = 0, ns = 0;
+    if (r < c.dim) {
+        const M m = (M)rep[r];
+        M rest = m;
+        while (rest) {
+            const int a = ctz_m<M>(rest);
+            rest &= rest - 1;
+            const M mp = m ^ ((M)1 << a);
+            for (int t = L.tstart[a]; t < L.tstart[a + 1]; ++t) {
+                const int b = L.tto[t];
+                if ((m >> b) & 1) continue;
+                ++ng;
+                const uint32_t e = table[colex_rank<M>(mp | ((M)1 << b), L.bin, c.bs)];
+                if (e != kNoRow) ++ns;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ng += __shfl_down(ng, off, kWave);
+        ns += __shfl_down(ns, off, kWave);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        red[0][threadIdx.x / kWave] = ng;
+        red[1][threadIdx.x / kWave] = ns;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0;
+        for (int i = 0; i < kMvBlock / kWave; ++i) {
+            a += red[0][i];
+            b += red[1][i];
+        }
+        partial[2 * blockIdx.x] = a;
+        partial[2 * blockIdx.x + 1] = b;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_prescale(const T* __restrict__ x, const uint8_t* __restrict__ stab,
+                                                  T* __restrict__ xh, int64_t n) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const T v = x[r];
+        const auto s = sqrt((decltype(v.x))stab[r]);
+        T o;
+        o.x = v.x * s;
+        o.y = v.y * s;
+        xh[r] = o;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gauge(double2* __restrict__ v, const double2* __restrict__ F,
+                                               int64_t n, int forward) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        double2 f = F[r];
+        if (!forward) f.y = -f.y;
+        v[r] = cmul(f, v[r]);
+    }
+}
+
+// F(rep_r) = prod_{i in rep_r} f(i)
+__global__ void __launch_bounds__(256) k_gauge_rows(const uint64_t* __restrict__ rep, const double2* __restrict__ f,
+                                                    int n_site, int64_t dim, double2* __restrict__ F) {
+    __shared__ double2 lf[64];
+    for (int i = threadIdx.x; i < n_site; i += blockDim.x) lf[i] = f[i];
+    __syncthreads();
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < dim; r += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t m = rep[r];
+        double2 a = make_double2(1.0, 0.0);
+        while (m) {
+            const int i = __builtin_ctzll(m);
+            m &= m - 1;
+            a = cmul(a, lf[i]);
+        }
+        F[r] = a;
+    }
+}
+
+// ------------------------------------------------------------------------
+// host launchers
+// ------------------------------------------------------------------------
+template <typename R>
+static MvConst make_const(const sed_sector* sec) {
+    const sed_catalog* cat = sec->cat;
+    MvConst c;
+    c.bin = cat->d_binom.p;
+    c.tstart = sec->d_term_start.p;
+    c.tto = sec->d_term_to.p;
+    c.dstart = sec->d_dens_start.p;
+    c.dj = sec->d_dens_j.p;
+    if (sizeof(R) == 8) {
+        c.tcoef = sec->d_term_coef64.p;
+        c.dv = sec->d_dens_v64.p;
+        c.chi = sec->d_chi64.p;
+    } else {
+        c.tcoef = sec->d_term_coef32.p;
+        c.dv = sec->d_dens_v32.p;
+        c.chi = sec->d_chi32.p;
+    }
+    c.n = cat->n_site;
+    c.k = cat->n_filled;
+    c.nG = cat->n_group;
+    c.bs = cat->bstride;
+    c.shift = sec->shift;
+    c.n_terms = sec->n_terms;
+    c.n_dens = sec->n_dens;
+    c.gmask = (1u << (sec->shift - 1)) - 1u;
+    c.dim = sec->dim;
+    return c;
+}
+
+template <typename R>
+size_t matvec_lds_bytes(const sed_sector* sec) {
+    MvConst c = make_const<R>(sec);
+    size_t off[9];
+    return mv_lds_layout<R>(c, off);
+}
+template size_t matvec_lds_bytes<float>(const sed_sector*);
+template size_t matvec_lds_bytes<double>(const sed_sector*);
+
+template <typename M, typename R, bool F>
+static int run_mv(sed_sector* sec, const typename C2<R>::T* xh, const typename C2<R>::T* x,
+                  typename C2<R>::T* y, hipStream_t s) {
+    MvConst c = make_const<R>(sec);
+    size_t off[9];
+    const size_t lds = mv_lds_layout<R>(c, off);
+    const int64_t nb = (sec->dim + kMvBlock - 1) / kMvBlock;
+    const int64_t grid = ((nb + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_matvec<M, R, F>), dim3((unsigned)grid), dim3(kMvBlock), lds, s, c, sec->d_rep.p,
+                       sec->d_stab.p, sec->d_table.p, xh, x, y, nb);
+    SED_HIP(hipGetLastError());
+    return SED_OK;
+}
+
+template <typename R>
+static int matvec_any(sed_sector* sec, const typename C2<R>::T* x, typename C2<R>::T* y, typename C2<R>::T* xh,
+                      hipStream_t s) {
+    using T = typename C2<R>::T;
+    if (!sec->has_terms) return fail(SED_ESTATE, "matvec before sed_sector_set_terms");
+    if (sec->dim == 0) return SED_OK;
+    const int64_t g = std::min<int64_t>((sec->dim + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(k_prescale<T>, dim3((unsigned)g), dim3(256), 0, s, x, sec->d_stab.p, xh, sec->dim);
+    SED_HIP(hipGetLastError());
+    const bool F = sec->cat->stats == SED_FERMIONIC;
+    if (sec->cat->n_site <= 32)
+        return F ? run_mv<uint32_t, R, true>(sec, xh, x, y, s) : run_mv<uint32_t, R, false>(sec, xh, x, y, s);
+    return F ? run_mv<uint64_t, R, true>(sec, xh, x, y, s) : run_mv<uint64_t, R, false>(sec, xh, x, y, s);
+}
+
+int launch_matvec64(sed_sector* sec, const double* x, double* y, double* xhat, hipStream_t s) {
+    return matvec_any<double>(sec, reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y),
+                              reinterpret_cast<double2*>(xhat), s);
+}
+int launch_matvec32(sed_sector* sec, const float* x, float* y, float* xhat, hipStream_t s) {
+    return matvec_any<float>(sec, reinterpret_cast<const float2*>(x), reinterpret_cast<float2*>(y),
+                             reinterpret_cast<float2*>(xhat), s);
+}
+
+// Main kernel only (the prescaled vector xhat already in place): for timing.
+int launch_matvec_kernel_only(sed_sector* sec, const void* xhat, const void* x, void* y, bool fp64, hipStream_t s) {
+    const bool F = sec->cat->stats == SED_FERMIONIC;
+    const bool small = sec->cat->n_site <= 32;
+    if (fp64) {
+        auto xh = reinterpret_cast<const double2*>(xhat);
+        auto xx = reinterpret_cast<const double2*>(x);
+        auto yy = reinterpret_cast<double2*>(y);
+        if (small) return F ? run_mv<uint32_t, double, true>(sec, xh, xx, yy, s) : run_mv<uint32_t, double, false>(sec, xh, xx, yy, s);
+        return F ? run_mv<uint64_t, double, true>(sec, xh, xx, yy, s) : run_mv<uint64_t, double, false>(sec, xh, xx, yy, s);
+    }
+    auto xh = reinterpret_cast<const float2*>(xhat);
+    auto xx = reinterpret_cast<const float2*>(x);
+    auto yy = reinterpret_cast<float2*>(y);
+    if (small) return F ? run_mv<uint32_t, float, true>(sec, xh, xx, yy, s) : run_mv<uint32_t, float, false>(sec, xh, xx, yy, s);
+    return F ? run_mv<uint64_t, float, true>(sec, xh, xx, yy, s) : run_mv<uint64_t, float, false>(sec, xh, xx, yy, s);
+}
+
+int launch_gauge64(sed_sector* sec, double* v, bool forward, hipStream_t s) {
+    if (!sec->has_gauge || sec->dim == 0) return SED_OK;
+    const int64_t g = std::min<int64_t>((sec->dim + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(k_gauge, dim3((unsigned)g), dim3(256), 0, s, reinterpret_cast<double2*>(v),
+                       reinterpret_cast<const double2*>(sec->d_gauge.p), sec->dim, forward ? 1 : 0);
+    SED_HIP(hipGetLastError());
+    return SED_OK;
+}
+
+int launch_gauge_rows(sed_sector* sec, const double* f_dev) {
+    const int64_t g = std::max<int64_t>(1, std::min<int64_t>((sec->dim + 255) / 256, 1 << 20));
+    hipLaunchKernelGGL(k_gauge_rows, dim3((unsigned)g), dim3(256), 0, sec->cat->ctx->stream, sec->d_rep.p,
+                       reinterpret_cast<const double2*>(f_dev), sec->cat->n_site, sec->dim,
+                       reinterpret_cast<double2*>(sec->d_gauge.p));
+    SED_HIP(hipGetLastError());
+    return SED_OK;
+}
+
+template <typename M>
+static int run_counts(sed_sector* sec, int64_t* gated, int64_t* ins) {
+    hipStream_t s = sec->cat->ctx->stream;
+    MvConst c = make_const<double>(sec);
+    size_t off[9];
+    const size_t lds = mv_lds_layout<double>(c, off);
+    const int64_t nb = std::max<int64_t>(1, (sec->dim + kMvBlock - 1) / kMvBlock);
+    DevBuf<unsigned long long> part;
+    SED_TRY(part.alloc(2 * nb));
+    hipLaunchKernelGGL(k_hop_counts<M>, dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p,
+                       sec->d_table.p, part.p);
+    SED_HIP(hipGetLastError());
+    std::vector<unsigned long long> h(2 * nb);
+    SED_HIP(hipMemcpyAsync(h.data(), part.p, part.bytes(), hipMemcpyDeviceToHost, s));
+    SED_HIP(hipStreamSynchronize(s));
+    unsigned long long a = 0, b = 0;
+    for (int64_t i = 0; i < nb; ++i) {
+        a += h[2 * i];
+        b += h[2 * i + 1];
+    }
+    *gated = (int64_t)a;
+    *ins = (int64_t)b;
+    return SED_OK;
+}
+
+int launch_hop_counts(sed_sector* sec, int64_t* gated, int64_t* in_sector) {
+    if (sec->dim == 0) {
+        *gated = *in_sector = 0;
+        return SED_OK;
+    }
+    if (sec->cat->n_site <= 32) return run_counts<uint32_t>(sec, gated, in_sector);
+    return run_counts<uint64_t>(sec, gated, in_sector);
+}
+
+int launch_dense(sed_sector* sec, double* H_dev, hipStream_t s) {
+    MvConst c = make_const<double>(sec);
+    size_t off[9];
+    const size_t lds = mv_lds_layout<double>(c, off);
+    SED_HIP(hipMemsetAsync(H_dev, 0, (size_t)sec->dim * sec->dim * 16, s));
+    const int64_t nb = (sec->dim + kMvBlock - 1) / kMvBlock;
+    const bool F = sec->cat->stats == SED_FERMIONIC;
+    auto H = reinterpret_cast<double2*>(H_dev);
+    if (sec->cat->n_site <= 32) {
+        if (F) hipLaunchKernelGGL((k_dense<uint32_t, true>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, H);
+        else hipLaunchKernelGGL((k_dense<uint32_t, false>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, H);
+    } else {
+        if (F) hipLaunchKernelGGL((k_dense<uint64_t, true>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, H);
+        else hipLaunchKernelGGL((k_dense<uint64_t, false>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, H);
+    }
+    SED_HIP(hipGetLastError());
+    return SED_OK;
+}
+
+}  // namespace sed

@@ -1,0 +1,367 @@
+"""Reference-named host API over the C ABI of ``libsed.so``.
+
+Mirrors the operator/plugin interface of the reference's matrix-free path
+(``src/symmetry_resolved_ed.jl``) with the same names, argument meaning and error
+behaviour (exceptions on failure):
+
+    build_symmetry_orbit_catalog   :371-487
+    build_symmetry_sector_basis    :544-556
+    hamiltonian_linear_operator    :891-897   -> (MatrixFreeHamiltonian, n)
+    MatrixFreeHamiltonian(x)       :880-884
+    diagonalize_block_matrixfree   :998-1030
+    build_ed_data                  :1460-1475
+    ed_scan_at_irrep_matrixfree!   :1092-1128   (ed_scan_at_irrep_matrixfree)
+    ed_scan! (flux mode)           :1297-1356   (ed_scan_flux, batched through the server)
+
+Every call runs on the GPU through the HIP library; there is no CPU path here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import PREC_FP32, PREC_FP32_REFINE, PREC_FP64, SolveStats, check, lib, ptr
+from .lattice import FERMIONIC, Irrep, Model, SymmetryGroup, model_group, term_arrays, translation_irreps, twisted
+
+__all__ = [
+    "Context", "SymmetryOrbitCatalog", "SymmetrySectorBasis", "MatrixFreeHamiltonian",
+    "build_symmetry_orbit_catalog", "build_symmetry_sector_basis", "hamiltonian_linear_operator",
+    "diagonalize_block_matrixfree", "build_ed_data", "ed_scan_at_irrep_matrixfree", "ed_scan_flux",
+    "SectorServer", "PREC_FP64", "PREC_FP32", "PREC_FP32_REFINE",
+]
+
+_default_ctx = None
+
+
+class Context:
+    """One GPU (device ordinal) with its HIP stream."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().sed_create(device, C.byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sed_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _c128(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.complex128)).view(np.float64)
+
+
+class SymmetryOrbitCatalog:
+    """Device-resident ``Symmetry_Orbit_Catalog`` (``src/symmetry_resolved_ed.jl:305-327``)."""
+
+    def __init__(self, ctx: Context, n_site: int, n_filled: int, group: SymmetryGroup, statistics: int):
+        self.ctx = ctx
+        self.n_site = n_site
+        self.n_filled = n_filled
+        self.symmetry_group = group
+        self.statistics = statistics
+        h = C.c_void_p()
+        perm = np.ascontiguousarray(group.perm, dtype=np.int32).reshape(-1)
+        check(lib().sed_catalog_build(ctx._h, n_site, n_filled, statistics, group.order, group.identity_idx,
+                                      perm, _c128(group.phases).reshape(-1), C.byref(h)))
+        self._h = h
+        nt, no = C.c_int64(), C.c_int64()
+        check(lib().sed_catalog_info(h, C.byref(nt), C.byref(no)))
+        self.n_total = nt.value
+        self.n_orbits = no.value
+
+    def tables(self):
+        """(rep_rank_table, rep_sym_table, rep_amp_table) in the reference's layout."""
+        rr = np.zeros(self.n_total, dtype=np.int32)
+        rs = np.zeros(self.n_total, dtype=np.int32)
+        ra = np.zeros(self.n_total, dtype=np.complex128)
+        check(lib().sed_catalog_copy(self._h, ptr(rr), ptr(rs), ptr(ra), None, None))
+        return rr, rs, ra
+
+    def orbits(self):
+        """(representative_mask_list, stabilizer_order_list, stabilizer g indices, phases)."""
+        nG = self.symmetry_group.order
+        reps = np.zeros(self.n_orbits, dtype=np.uint64)
+        so = np.zeros(self.n_orbits, dtype=np.int64)
+        check(lib().sed_catalog_copy(self._h, None, None, None, ptr(reps), ptr(so)))
+        gi = np.zeros((self.n_orbits, nG), dtype=np.int32)
+        gp = np.zeros((self.n_orbits, nG), dtype=np.complex128)
+        check(lib().sed_catalog_stabilizers(self._h, ptr(gi), ptr(gp)))
+        return reps, so, gi, gp
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sed_catalog_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_symmetry_orbit_catalog(*, n_site: int, n_filled: int, symmetry_group: SymmetryGroup,
+                                 particle_statistics: int, ctx: Optional[Context] = None) -> SymmetryOrbitCatalog:
+    return SymmetryOrbitCatalog(ctx or default_context(), n_site, n_filled, symmetry_group, particle_statistics)
+
+
+class SymmetrySectorBasis:
+    """``Symmetry_Sector_Basis`` of one 1D irrep (``src/symmetry_resolved_ed.jl:522-528``)."""
+
+    def __init__(self, catalog: SymmetryOrbitCatalog, irrep: Irrep, atol: float = 1e-12):
+        self.catalog = catalog
+        self.irrep = irrep
+        chi = _c128(irrep.values).reshape(-1)
+        h = C.c_void_p()
+        d = C.c_int64()
+        check(lib().sed_sector_build(catalog._h, chi, atol, C.byref(h), C.byref(d)))
+        self._h = h
+        self.dim = d.value
+        self._terms_set = False
+
+    @property
+    def representative_mask_list(self) -> np.ndarray:
+        reps = np.zeros(self.dim, dtype=np.uint64)
+        check(lib().sed_sector_basis(self._h, ptr(reps), None))
+        return reps
+
+    @property
+    def stabilizer_order_list(self) -> np.ndarray:
+        so = np.zeros(self.dim, dtype=np.int64)
+        check(lib().sed_sector_basis(self._h, None, ptr(so)))
+        return so
+
+    def set_terms(self, bilinear_terms, density_terms):
+        i_from = np.array([a for a, _, _ in bilinear_terms], dtype=np.int32)
+        i_to = np.array([b for _, b, _ in bilinear_terms], dtype=np.int32)
+        t = np.array([c for _, _, c in bilinear_terms], dtype=np.complex128)
+        di = np.array([a for a, _, _ in density_terms], dtype=np.int32)
+        dj = np.array([b for _, b, _ in density_terms], dtype=np.int32)
+        v = np.array([c for _, _, c in density_terms], dtype=np.complex128)
+        self.set_term_arrays(i_from, i_to, t, di, dj, v)
+
+    def set_term_arrays(self, i_from, i_to, t, di, dj, v):
+        i_from = np.ascontiguousarray(i_from, dtype=np.int32)
+        i_to = np.ascontiguousarray(i_to, dtype=np.int32)
+        di = np.ascontiguousarray(di, dtype=np.int32)
+        dj = np.ascontiguousarray(dj, dtype=np.int32)
+        check(lib().sed_sector_set_terms(self._h, len(i_from), i_from, i_to, _c128(t).reshape(-1) if len(t) else
+                                         np.zeros(2), len(di), di, dj, _c128(v).reshape(-1) if len(v) else
+                                         np.zeros(2)))
+        self._terms_set = True
+
+    def hop_counts(self) -> Tuple[int, int]:
+        g, s = C.c_int64(), C.c_int64()
+        check(lib().sed_sector_hop_counts(self._h, C.byref(g), C.byref(s)))
+        return g.value, s.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sed_sector_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_symmetry_sector_basis(catalog: SymmetryOrbitCatalog, irrep: Irrep, atol: float = 1e-12) -> SymmetrySectorBasis:
+    return SymmetrySectorBasis(catalog, irrep, atol)
+
+
+class MatrixFreeHamiltonian:
+    """Callable sector operator ``H(x) -> y`` (``src/symmetry_resolved_ed.jl:767-791, 880-884``)."""
+
+    def __init__(self, basis: SymmetrySectorBasis, bilinear_terms, density_terms, precision: int = PREC_FP64):
+        self.basis = basis
+        basis.set_terms(bilinear_terms, density_terms)
+        self.n = basis.dim
+        self.precision = precision
+
+    def __call__(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        if x.shape != (self.n,):
+            raise ValueError(f"expected a vector of length {self.n}")
+        y = np.zeros(self.n, dtype=np.complex128)
+        check(lib().sed_matvec(self.basis._h, x.view(np.float64), y.view(np.float64), self.precision))
+        return y
+
+    def benchmark(self, precision: int = PREC_FP64, n_calls: int = 10) -> float:
+        """Mean device time (ms) of the H|psi> kernel over n_calls back-to-back launches."""
+        ms = C.c_double()
+        check(lib().sed_matvec_benchmark(self.basis._h, precision, n_calls, C.byref(ms)))
+        return ms.value
+
+
+def hamiltonian_linear_operator(basis: SymmetrySectorBasis, bilinear_terms, density_terms, cmap=None):
+    """``(H_op, n)`` as the reference returns it; ``cmap`` is accepted and unused (the
+    representative tables live on the device)."""
+    H = MatrixFreeHamiltonian(basis, bilinear_terms, density_terms)
+    return H, H.n
+
+
+def diagonalize_block_matrixfree(H_op: MatrixFreeHamiltonian, n: int, nev: int = 5, *, tol: float = 1e-10,
+                                 seed: int = 1234, maxiter: int = 300, precision: int = PREC_FP64,
+                                 return_vectors: bool = True, stats: Optional[dict] = None):
+    """Lowest ``nev`` eigenpairs ``(vals, vecs)`` of a sector operator.
+
+    Raises ``SedError`` (code ENOCONV) when fewer than ``nev`` pairs converge, as the
+    reference errors (``:1023``)."""
+    if n == 0:
+        return np.zeros(0), np.zeros((0, 0), dtype=np.complex128)
+    k = min(nev, n)
+    vals = np.zeros(k, dtype=np.float64)
+    vecs = np.zeros((k, n), dtype=np.complex128) if return_vectors else None
+    st = SolveStats()
+    rc = lib().sed_solve(H_op.basis._h, k, tol, seed, maxiter, precision, vals, ptr(vecs) if vecs is not None else None,
+                         C.byref(st))
+    if stats is not None:
+        stats.update(st.as_dict())
+    check(rc)
+    return vals, (vecs.T.copy() if vecs is not None else None)
+
+
+def sector_density(H_op: MatrixFreeHamiltonian, which: int = 0) -> np.ndarray:
+    """Site densities of eigenvector ``which`` of the last solve on this sector."""
+    d = np.zeros(H_op.basis.catalog.n_site, dtype=np.float64)
+    check(lib().sed_sector_density(H_op.basis._h, which, d))
+    return d
+
+
+@dataclass
+class EDData:
+    """``Symmetry_Resolved_ED_Data`` (``src/symmetry_resolved_ed.jl:1051-1060``)."""
+    model: Model
+    n_filled: int
+    symmetry_group: SymmetryGroup
+    irrep_list: List[Irrep]
+    orbit_catalog: SymmetryOrbitCatalog
+    sector_dims: List[int]
+    ed_scan_res: Dict[int, Tuple[np.ndarray, np.ndarray]] = field(default_factory=dict)
+
+
+def build_ed_data(model: Model, symmetry_group: Optional[SymmetryGroup] = None,
+                  ctx: Optional[Context] = None) -> EDData:
+    G = symmetry_group if symmetry_group is not None else model_group(model)
+    irreps = translation_irreps(G, model.lattice)
+    cat = build_symmetry_orbit_catalog(n_site=model.n_site, n_filled=model.n_filled, symmetry_group=G,
+                                       particle_statistics=model.statistics, ctx=ctx)
+    return EDData(model, model.n_filled, G, irreps, cat, [0] * len(irreps))
+
+
+def ed_scan_at_irrep_matrixfree(irrep_label, ed_data: EDData, nev: int = 5, **solve_kw):
+    """Per-sector driver (``src/symmetry_resolved_ed.jl:1092-1128``)."""
+    idx = next(i for i, ir in enumerate(ed_data.irrep_list) if tuple(ir.label) == tuple(irrep_label))
+    basis = build_symmetry_sector_basis(ed_data.orbit_catalog, ed_data.irrep_list[idx])
+    ed_data.sector_dims[idx] = basis.dim
+    if basis.dim == 0:
+        ed_data.ed_scan_res[idx] = (np.zeros(0), np.zeros((0, 0), dtype=np.complex128))
+        return ed_data.ed_scan_res[idx]
+    H, n = hamiltonian_linear_operator(basis, ed_data.model.bilinear_terms, ed_data.model.density_terms)
+    vals, vecs = diagonalize_block_matrixfree(H, n, nev=nev, **solve_kw)
+    ed_data.ed_scan_res[idx] = (vals, vecs)
+    return vals, vecs
+
+
+class SectorServer:
+    """Continuous-batching service of sector solves (``sed_server_*``)."""
+
+    def __init__(self, ctx: Optional[Context] = None, hbm_budget_bytes: int = 0, max_slots: int = 8):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        check(lib().sed_server_create(self.ctx._h, int(hbm_budget_bytes), int(max_slots), C.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, basis: SymmetrySectorBasis, nev: int, tol: float = 1e-10, seed: int = 1234,
+               maxiter: int = 300, precision: int = PREC_FP64) -> int:
+        rid = C.c_int64()
+        check(lib().sed_server_submit(self._h, basis._h, nev, tol, seed, maxiter, precision, C.byref(rid)))
+        self._keep[rid.value] = (basis, nev)
+        return rid.value
+
+    def run(self, max_steps: int = 0) -> int:
+        nd = C.c_int64()
+        check(lib().sed_server_run(self._h, max_steps, C.byref(nd)))
+        return nd.value
+
+    def result(self, rid: int):
+        """(status, evals, stats dict, latency seconds); status 0 done, 1 pending, 2 running, <0 error."""
+        basis, nev = self._keep[rid]
+        st = C.c_int()
+        ev = np.full(nev, np.nan)
+        stats = SolveStats()
+        lat = C.c_double(float("nan"))
+        check(lib().sed_server_result(self._h, rid, C.byref(st), ev, C.byref(stats), C.byref(lat)))
+        return st.value, ev, stats.as_dict(), lat.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sed_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ed_scan_flux(model: Model, thetas: Sequence[Tuple[float, float]], nev: int = 3, *,
+                 sectors: Optional[Sequence[Tuple[int, int]]] = None, precision: int = PREC_FP64,
+                 tol: float = 1e-10, seed: int = 1234, max_slots: int = 8, ctx: Optional[Context] = None):
+    """Flux scan (``ed_scan!`` with ``twisted_phases_over_2pi_list``, ``:1320-1355``): every
+    (twist, momentum) sector solve is submitted to one continuous-batching server.
+
+    Returns {(theta, k_label): evals}.  The orbit partition does not depend on the twist,
+    but each twist gets its own group phases, catalog and sectors as in the reference."""
+    ctx = ctx or default_context()
+    srv = SectorServer(ctx, max_slots=max_slots)
+    keep = []
+    ids = {}
+    for th in thetas:
+        m = twisted(model, th)
+        G = model_group(m, th)
+        irreps = translation_irreps(G, m.lattice)
+        cat = build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                           particle_statistics=m.statistics, ctx=ctx)
+        for ir in irreps:
+            if sectors is not None and tuple(ir.label) not in set(map(tuple, sectors)):
+                continue
+            b = build_symmetry_sector_basis(cat, ir)
+            if b.dim == 0:
+                continue
+            b.set_terms(m.bilinear_terms, m.density_terms)
+            keep.append(b)
+            ids[(tuple(th), tuple(ir.label))] = srv.submit(b, min(nev, b.dim), tol, seed, 300, precision)
+        keep.append(cat)
+    srv.run(0)
+    out = {}
+    for key, rid in ids.items():
+        st, ev, stats, lat = srv.result(rid)
+        if st != 0:
+            raise _lib.SedError(st, f"sector {key} failed")
+        out[key] = ev
+    srv.close()
+    return out

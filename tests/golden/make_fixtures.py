@@ -1,0 +1,64 @@
+"""Extract the reference's own stored results into a small JSON fixture.
+
+Reads (as data, in this container) the benchmark CSVs the reference keeps under
+benchmark/benchmark_data/ and the printed outputs of doc/design.ipynb and
+doc/charge_pump.ipynb, and writes tests/golden/reference_values.json.  Nothing of
+the reference is imported or run.
+"""
+import csv
+import json
+import os
+import re
+
+REF = "/root/reference"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_values.json")
+
+
+def csv_rows(name):
+    with open(os.path.join(REF, "benchmark", "benchmark_data", name)) as f:
+        return list(csv.DictReader(f))
+
+
+def main():
+    data = {"benchmark_raw": [], "benchmark_raw_py_scan": [], "benchmark_raw_single": []}
+    for key in data:
+        for r in csv_rows(key + ".csv"):
+            data[key].append({
+                "model": r["model"], "label": r["label"], "n_site": int(r["n_site"]),
+                "n_filled": int(r["n_filled"]), "full_dim": int(r["full_dim"]),
+                "n_orbits": int(r["n_orbits"]), "n_group": int(r["n_group"]),
+                "sector_dim": int(r["sector_dim"]), "mode": r["mode"], "energy": float(r["energy"]),
+            })
+    nb = json.load(open(os.path.join(REF, "doc", "design.ipynb")))
+    outs = {}
+    for i, c in enumerate(nb["cells"]):
+        txt = ""
+        for o in c.get("outputs", []):
+            t = o.get("text") or o.get("data", {}).get("text/plain") or ""
+            txt += "".join(t)
+        outs[i] = txt
+    cell22 = outs[22]
+    ranks = [int(x) for x in re.findall(r"rank = (\d+)", cell22)]
+    masks = [int(x) for x in re.findall(r"^\s+(\d+) = ", cell22, re.M)]
+    cell26 = outs[26]
+    cell43 = outs[43]
+    data["design_notebook"] = {
+        "colex_n4_k2": {"masks": masks, "ranks": ranks},
+        "lin_index_mismatches_12_3": int(re.search(r"mismatches over 220 masks: (\d+)", outs[24]).group(1)),
+        "haldane_2x3": {
+            "n_orbits": int(re.search(r"n_orbits = (\d+)", cell26).group(1)),
+            "full_basis": int(re.search(r"C\(12,3\) = (\d+)", cell26).group(1)),
+            "unset": int(re.search(r"unset = (\d+)", cell26).group(1)),
+            "rep4_mask": int(re.search(r"get_canonical\(rep\[4\] = (\d+)\)", cell26).group(1)),
+            "rep4_canonical": [int(x) for x in re.search(r"\(repr = (\d+), g_idx = (\d+)", cell26).groups()],
+            "sector_dim": int(re.search(r"sector dim = (\d+)", cell43).group(1)),
+            "nnz_H": int(re.search(r"nnz\(H\) = (\d+)", cell43).group(1)),
+            "table_entries": int(re.search(r"projection table entries = (\d+)", cell43).group(1)),
+        },
+    }
+    json.dump(data, open(OUT, "w"), indent=1)
+    print("wrote", OUT)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,224 @@
+"""Parity of the HIP path (through the C ABI of libsed.so) with the CPU oracle and the
+reference's stored results.
+
+Bars: bit-exact for integer tables (rep index, group index, sign, representatives,
+stabilizer orders); rep amplitudes exact at zero twist and within 1e-13 otherwise
+(the reference forms them with @fastmath products); matvec within 1e-12 relative
+(FP64) and 2e-6 relative (FP32); eigenvalues within 1e-10 relative.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from realspace_exactdiagonalization_amd import lattice as L
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ed():
+    from realspace_exactdiagonalization_amd import ed as E
+    return E
+
+
+@pytest.fixture(scope="module")
+def ctx(ed):
+    c = ed.Context(0)
+    yield c
+    c.close()
+
+
+CASES = {
+    "hald2x3": (lambda: L.haldane_honeycomb(2, 3, n_filled=3), (0.0, 0.0)),
+    "hub2x3": (lambda: L.hubbard_square(2, 3, U=4.0, n_filled=6), (0.0, 0.0)),
+    "heis14": (lambda: L.heisenberg_chain(14), (0.0, 0.0)),
+    "checker3x2_f": (lambda: L.checkerboard(3, 2, n_filled=4, V1=1.0, V2=0.5), (0.0, 0.0)),
+    "checker3x2_f_flux": (lambda: L.checkerboard(3, 2, n_filled=4, V1=1.0, V2=0.5), (0.23, 0.17)),
+    "checker3x2_b_flux": (lambda: L.checkerboard(3, 2, n_filled=4, statistics=L.BOSONIC), (0.23, 0.17)),
+    "hald3x2_flux": (lambda: L.haldane_honeycomb(3, 2, n_filled=3, V1=0.3), (0.4, 0.0)),
+    "checker3x4_f": (lambda: L.checkerboard(3, 4, n_filled=8), (0.0, 0.0)),
+}
+
+
+def make(name):
+    f, th = CASES[name]
+    m = L.twisted(f(), th)
+    return m, L.model_group(m, th)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_catalog_tables_match_reference_layout(ed, ctx, oracle_mod, name):
+    m, G = make(name)
+    print(f"[catalog] {name} N={m.n_site} k={m.n_filled} |G|={G.order}", flush=True)
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, m.statistics == L.FERMIONIC)
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    assert gc.n_total == oc.n_total and gc.n_orbits == oc.n_orbits
+    orr, ors, ora = oc.tables()
+    grr, grs, gra = gc.tables()
+    assert np.array_equal(orr, grr)
+    assert np.array_equal(ors, grs)
+    exact = all(abs(p - 1) == 0 for p in G.phases.reshape(-1))
+    if exact:
+        assert np.array_equal(ora, gra)
+    else:
+        assert np.abs(ora - gra).max() < 1e-13
+    oreps, oso, ogi, ogp = oc.orbits()
+    greps, gso, ggi, ggp = gc.orbits()
+    assert np.array_equal(oreps, greps) and np.array_equal(oso, gso) and np.array_equal(ogi, ggi)
+    assert np.abs(ogp - ggp).max() < 1e-13
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_sector_basis_and_matvec(ed, ctx, oracle_mod, name):
+    m, G = make(name)
+    print(f"[sector/matvec] {name}", flush=True)
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, m.statistics == L.FERMIONIC)
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    rng = np.random.default_rng(7)
+    for ir in L.translation_irreps(G, m.lattice):
+        os_ = oracle_mod.Sector(oc, ir.values)
+        gb = ed.build_symmetry_sector_basis(gc, ir)
+        assert gb.dim == os_.dim
+        if os_.dim == 0:
+            continue
+        oreps, ostab = os_.basis()
+        assert np.array_equal(oreps, gb.representative_mask_list)
+        assert np.array_equal(ostab, gb.stabilizer_order_list)
+        os_.set_terms(*L.term_arrays(m))
+        H = ed.MatrixFreeHamiltonian(gb, m.bilinear_terms, m.density_terms)
+        x = rng.standard_normal(gb.dim) + 1j * rng.standard_normal(gb.dim)
+        y_ref = os_.apply_table(x, 1)
+        y = H(x)
+        scale = np.abs(y_ref).max() + 1e-300
+        assert np.abs(y - y_ref).max() <= 1e-12 * scale
+        H.precision = ed.PREC_FP32
+        y32 = H(x)
+        assert np.abs(y32 - y_ref).max() <= 2e-6 * scale * max(1, math.sqrt(len(m.bilinear_terms) / 10))
+        # flux_regression.jl:32-34 probe vector x = 1:n + i
+        H.precision = ed.PREC_FP64
+        xr = np.arange(1, gb.dim + 1) + 1j
+        assert np.abs(H(xr) - os_.apply_table(xr, 1)).max() <= 1e-12 * np.abs(os_.apply_table(xr, 1)).max()
+
+
+def test_matvec_deterministic(ed, ctx):
+    m, G = make("checker3x4_f")
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    gb = ed.build_symmetry_sector_basis(gc, L.translation_irreps(G, m.lattice)[0])
+    H = ed.MatrixFreeHamiltonian(gb, m.bilinear_terms, m.density_terms)
+    x = np.random.default_rng(3).standard_normal(gb.dim) + 0.5j
+    a, b = H(x), H(x)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["hald2x3", "hub2x3", "heis14", "checker3x2_f_flux", "hald3x2_flux"])
+@pytest.mark.parametrize("precision", [0, 2])
+def test_eigenvalues_vs_oracle_all_sectors(ed, ctx, oracle_mod, name, precision):
+    m, G = make(name)
+    print(f"[eigs] {name} precision={precision}", flush=True)
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, m.statistics == L.FERMIONIC)
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    for ir in L.translation_irreps(G, m.lattice):
+        os_ = oracle_mod.Sector(oc, ir.values)
+        if os_.dim == 0:
+            continue
+        os_.set_terms(*L.term_arrays(m))
+        nev = min(3, os_.dim)
+        ref = oracle_mod.lowest_eigenvalues(os_, nev)
+        gb = ed.build_symmetry_sector_basis(gc, ir)
+        H, n = ed.hamiltonian_linear_operator(gb, m.bilinear_terms, m.density_terms)
+        vals, vecs = ed.diagonalize_block_matrixfree(H, n, nev=nev, precision=precision)
+        assert np.abs(vals - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max()), (ir.label, vals, ref)
+        # returned vectors are eigenvectors of the reference-basis operator
+        for j in range(nev):
+            r = H(vecs[:, j]) - vals[j] * vecs[:, j]
+            assert np.linalg.norm(r) < 1e-6
+
+
+GOLDEN = [("Heisenberg", "N=20"), ("Heisenberg", "N=24"), ("Haldane_Boson", "3×4"), ("Haldane_Boson", "2×7"),
+          ("Hubbard_Fermion", "2×5"), ("Hubbard_Fermion", "3×4")]
+
+
+@pytest.mark.parametrize("model,label", GOLDEN)
+def test_lowest_energy_vs_reference_csv(ed, ctx, golden, model, label):
+    from tests.test_oracle import csv_model
+    print(f"[golden] {model} {label}", flush=True)
+    rows = [r for r in golden["benchmark_raw"] if r["model"] == model and r["label"] == label]
+    m = csv_model(model, label)
+    data = ed.build_ed_data(m, ctx=ctx)
+    assert data.orbit_catalog.n_orbits == rows[0]["n_orbits"]
+    for prec in (0, 2):
+        vals, _ = ed.ed_scan_at_irrep_matrixfree(data.irrep_list[0].label, data, nev=1, precision=prec,
+                                                 return_vectors=False)
+        assert data.sector_dims[0] == rows[0]["sector_dim"]
+        for r in rows:
+            assert abs(vals[0] - r["energy"]) <= 1e-10 * abs(r["energy"]), (prec, vals[0], r["energy"])
+
+
+def test_site_densities(ed, ctx, oracle_mod):
+    """Sector densities vs the oracle eigenvector (orbit expansion), plus I4: translation
+    invariance per sublattice and sum = particle number."""
+    m, G = make("checker3x2_f_flux")
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    ir = L.translation_irreps(G, m.lattice)[0]
+    gb = ed.build_symmetry_sector_basis(gc, ir)
+    H, n = ed.hamiltonian_linear_operator(gb, m.bilinear_terms, m.density_terms)
+    vals, vecs = ed.diagonalize_block_matrixfree(H, n, nev=2)
+    d = ed.sector_density(H, 0)
+    assert abs(d.sum() - m.n_filled) < 1e-10
+    for s in (1, 2):
+        sub = [d[i - 1] for i in range(1, m.n_site + 1) if m.lattice.site_list[i - 1][1] == s]
+        assert np.ptp(sub) < 1e-10
+    # oracle: <n_i> = sum_r |c_r|^2 / |G| * #{g : i in g(rep_r)}
+    reps = gb.representative_mask_list
+    c = vecs[:, 0]
+    ref = np.zeros(m.n_site)
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, True)
+    for r, mask in enumerate(reps):
+        for g in range(1, G.order + 1):
+            img, _ = oc.apply_operation(int(mask), g)
+            for i in range(m.n_site):
+                if (img >> i) & 1:
+                    ref[i] += abs(c[r]) ** 2 / G.order
+    assert np.abs(ref - d).max() < 1e-12
+
+
+def test_server_results_alone_and_in_mix(ed, ctx):
+    """Each solve gives the same eigenvalues alone and when batched with others."""
+    m0 = L.haldane_honeycomb(3, 4)
+    bases = []
+    keep = []
+    for th in [(0.0, 0.0), (0.25, 0.0), (0.5, 0.0)]:
+        m = L.twisted(m0, th)
+        G = L.model_group(m, th)
+        cat = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                              particle_statistics=m.statistics, ctx=ctx)
+        keep.append(cat)
+        for ir in L.translation_irreps(G, m.lattice)[:4]:
+            b = ed.build_symmetry_sector_basis(cat, ir)
+            b.set_terms(m.bilinear_terms, m.density_terms)
+            bases.append(b)
+    alone = []
+    for b in bases:
+        H = ed.MatrixFreeHamiltonian.__new__(ed.MatrixFreeHamiltonian)
+        H.basis, H.n, H.precision = b, b.dim, ed.PREC_FP64
+        alone.append(ed.diagonalize_block_matrixfree(H, b.dim, nev=3, return_vectors=False)[0])
+    srv = ed.SectorServer(ctx, max_slots=4)
+    ids = [srv.submit(b, 3, precision=p) for b in bases for p in (ed.PREC_FP64,)]
+    srv.run(0)
+    for rid, ref in zip(ids, alone):
+        st, ev, stats, lat = srv.result(rid)
+        assert st == 0
+        assert np.abs(ev - ref).max() <= 1e-10 * np.abs(ref).max()
+        assert lat >= 0
+    ids32 = [srv.submit(b, 3, precision=ed.PREC_FP32_REFINE) for b in bases]
+    srv.run(0)
+    for rid, ref in zip(ids32, alone):
+        st, ev, stats, lat = srv.result(rid)
+        assert st == 0 and np.abs(ev - ref).max() <= 1e-10 * np.abs(ref).max()
+    srv.close()
