@@ -66,6 +66,11 @@ def lib():
         L.oc_canonical_representative.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32), _f64p]
         L.oc_apply_operation.restype = C.c_uint64
         L.oc_apply_operation.argtypes = [C.c_void_p, C.c_uint64, C.c_int, _f64p]
+        L.oc_catalog_wrap.restype = C.c_void_p
+        L.oc_catalog_wrap.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _f64p, C.c_int, C.c_int64, _u64p, _i32p,
+                                      _i32p, _f64p]
+        L.oc_sector_from_basis.restype = C.c_void_p
+        L.oc_sector_from_basis.argtypes = [C.c_void_p, _f64p, C.c_int64, _u64p, _i64p]
         L.oc_sector_build.restype = C.c_void_p
         L.oc_sector_build.argtypes = [C.c_void_p, _f64p, C.c_double]
         L.oc_sector_free.argtypes = [C.c_void_p]
@@ -174,13 +179,39 @@ class Catalog:
         return int(r), complex(a[0], a[1])
 
 
+class WrappedCatalog(Catalog):
+    """Catalog over externally produced tables in the reference layout (borrowed)."""
+
+    def __init__(self, n_site, n_filled, group, fermionic, reps, rep_rank, rep_sym, rep_amp):
+        self.group = group
+        self.n_site = n_site
+        self.n_filled = n_filled
+        self.fermionic = bool(fermionic)
+        self._arrays = (np.ascontiguousarray(reps, dtype=np.uint64), np.ascontiguousarray(rep_rank, dtype=np.int32),
+                        np.ascontiguousarray(rep_sym, dtype=np.int32),
+                        np.ascontiguousarray(rep_amp, dtype=np.complex128))
+        r, rr, rs, ra = self._arrays
+        perm = np.ascontiguousarray(group.perm, dtype=np.int32)
+        self._h = lib().oc_catalog_wrap(n_site, n_filled, group.order, perm, _c128(group.phases),
+                                        1 if fermionic else 0, len(r), r, rr, rs, ra.view(np.float64))
+        if not self._h:
+            raise RuntimeError("oracle catalog wrap failed")
+        self.n_orbits = len(r)
+        self.n_total = len(rr)
+
+
 class Sector:
     """``build_symmetry_sector_basis`` + projection table + matvecs."""
 
-    def __init__(self, catalog: Catalog, chi, atol: float = 1e-12):
+    def __init__(self, catalog: Catalog, chi, atol: float = 1e-12, basis=None):
         self.catalog = catalog
         self.chi = np.asarray(chi, dtype=np.complex128)
-        self._h = lib().oc_sector_build(catalog._h, _c128(self.chi), atol)
+        if basis is None:
+            self._h = lib().oc_sector_build(catalog._h, _c128(self.chi), atol)
+        else:
+            reps = np.ascontiguousarray(basis[0], dtype=np.uint64)
+            stab = np.ascontiguousarray(basis[1], dtype=np.int64)
+            self._h = lib().oc_sector_from_basis(catalog._h, _c128(self.chi), len(reps), reps, stab)
         self.dim = int(lib().oc_sector_dim(self._h))
         self.nnz = None
         self._terms = None
@@ -206,10 +237,14 @@ class Sector:
         r = lib().oc_project_to_unnormalized_sector(self._h, int(m), c)
         return None if r == 0 else (int(r), complex(c[0], c[1]))
 
-    def set_terms(self, i_from, i_to, t, di, dj, V):
+    def set_terms(self, i_from, i_to, t, di, dj, V, build_table: bool = True):
+        """Attach the term lists; with build_table the projection table of
+        build_matrixfree_projection_table is built (needed by apply_table)."""
         self._terms = (np.ascontiguousarray(i_from, dtype=np.int32), np.ascontiguousarray(i_to, dtype=np.int32),
                        _c128(t), np.ascontiguousarray(di, dtype=np.int32), np.ascontiguousarray(dj, dtype=np.int32),
                        _c128(V))
+        if not build_table:
+            return None
         a, b, tt, c, d, vv = self._terms
         self.nnz = int(lib().oc_table_build(self._h, len(a), a, b, tt, len(c), c, d, vv))
         return self.nnz

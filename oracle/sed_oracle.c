@@ -183,6 +183,7 @@ typedef struct {
     int32_t* rep_rank;    /* n_total, 1-based orbit index, 0 = unset */
     int32_t* rep_sym;     /* n_total, 1-based inverse op index */
     cplx* rep_amp;        /* n_total */
+    int borrowed;         /* tables/reps owned by the caller (oc_catalog_wrap) */
 } oc_catalog;
 
 static void cat_push(oc_catalog* c, mask_t m, int64_t so, const int32_t* gi, const cplx* gp) {
@@ -206,9 +207,38 @@ static void cat_push(oc_catalog* c, mask_t m, int64_t so, const int32_t* gi, con
 void oc_catalog_free(oc_catalog* c) {
     if (!c) return;
     free(c->G.perm); free(c->G.phases); free(c->G.inv);
-    free(c->reps); free(c->stab_order); free(c->stab_gidx); free(c->stab_phase);
-    free(c->rep_rank); free(c->rep_sym); free(c->rep_amp);
+    if (!c->borrowed) {
+        free(c->reps); free(c->stab_order); free(c->stab_gidx); free(c->stab_phase);
+        free(c->rep_rank); free(c->rep_sym); free(c->rep_amp);
+    }
     free(c);
+}
+
+/* A catalog over tables produced elsewhere (borrowed, not copied): lets the CPU
+ * baseline time the reference's matvec on sizes whose serial catalog build would
+ * take minutes.  Only the lookup tables and the representative list are needed by
+ * get_canonical / project_to_sector. */
+oc_catalog* oc_catalog_wrap(int n_site, int n_filled, int n_group, const int32_t* perm, const double* phases,
+                            int fermionic, int64_t n_orbits, uint64_t* reps, int32_t* rep_rank, int32_t* rep_sym,
+                            double* rep_amp) {
+    binom_init();
+    oc_catalog* c = calloc(1, sizeof(oc_catalog));
+    c->G.n_site = n_site; c->G.n_group = n_group; c->G.fermionic = fermionic;
+    c->G.perm = malloc((size_t)n_group * n_site * sizeof(int32_t));
+    c->G.phases = malloc((size_t)n_group * n_site * sizeof(cplx));
+    c->G.inv = malloc((size_t)n_group * sizeof(int32_t));
+    memcpy(c->G.perm, perm, (size_t)n_group * n_site * sizeof(int32_t));
+    memcpy(c->G.phases, phases, (size_t)n_group * n_site * sizeof(cplx));
+    if (oc_inverse_indices(n_site, n_group, perm, phases, c->G.inv) != 0) { oc_catalog_free(c); return NULL; }
+    c->n_filled = n_filled;
+    c->n_total = binom(n_site, n_filled);
+    c->n_orbits = n_orbits;
+    c->reps = reps;
+    c->rep_rank = rep_rank;
+    c->rep_sym = rep_sym;
+    c->rep_amp = (cplx*)rep_amp;
+    c->borrowed = 1;
+    return c;
 }
 
 oc_catalog* oc_catalog_build(int n_site, int n_filled, int n_group, const int32_t* perm,
@@ -357,6 +387,21 @@ oc_sector* oc_sector_build(const oc_catalog* c, const double* chi, double atol) 
 }
 
 int64_t oc_sector_dim(const oc_sector* s) { return s->dim; }
+
+/* A sector over a given basis (representatives ascending, stabilizer orders). */
+oc_sector* oc_sector_from_basis(const oc_catalog* c, const double* chi, int64_t dim, const uint64_t* reps,
+                                const int64_t* stab) {
+    oc_sector* s = calloc(1, sizeof(oc_sector));
+    s->cat = c;
+    s->chi = malloc(c->G.n_group * sizeof(cplx));
+    memcpy(s->chi, chi, c->G.n_group * sizeof(cplx));
+    s->dim = dim;
+    s->reps = malloc((dim + 1) * sizeof(mask_t));
+    s->stab = malloc((dim + 1) * sizeof(int64_t));
+    memcpy(s->reps, reps, dim * sizeof(mask_t));
+    memcpy(s->stab, stab, dim * sizeof(int64_t));
+    return s;
+}
 
 void oc_sector_basis(const oc_sector* s, uint64_t* reps, int64_t* stab) {
     memcpy(reps, s->reps, s->dim * sizeof(mask_t));

@@ -96,6 +96,12 @@ class SymmetryOrbitCatalog:
         check(lib().sed_catalog_copy(self._h, ptr(rr), ptr(rs), ptr(ra), None, None))
         return rr, rs, ra
 
+    def representatives(self) -> np.ndarray:
+        """representative_mask_list (ascending)."""
+        reps = np.zeros(self.n_orbits, dtype=np.uint64)
+        check(lib().sed_catalog_copy(self._h, None, None, None, ptr(reps), None))
+        return reps
+
     def orbits(self):
         """(representative_mask_list, stabilizer_order_list, stabilizer g indices, phases)."""
         nG = self.symmetry_group.order
