@@ -83,10 +83,24 @@ def main():
     ap.add_argument("--workload", default="checker4x6")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28                      # 2^28 floats = 1 GiB
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n,
+                          "dispatches": 3}))
+        return
 
     from realspace_exactdiagonalization_amd import ed as E
     from realspace_exactdiagonalization_amd import lattice as L
@@ -117,9 +131,11 @@ def main():
         ms = H.benchmark(prec, args.steps)
         wall = time.time() - w0
         B = algorithmic_bytes(dim, gated, ins, s)
-        res[name] = {"ms": ms, "bytes": B, "gbs": B / (ms * 1e-3) / 1e9, "wall_s": wall}
+        res[name] = {"ms": ms, "bytes": B, "gbs": B / (ms * 1e-3) / 1e9, "wall_s": wall,
+                     "shape": basis.shape(prec)}
         print(f"[bench] {name}: {ms:.3f} ms/matvec  {B/1e9:.2f} GB algorithmic  "
-              f"{res[name]['gbs']:.0f} GB/s ({100*res[name]['gbs']/PEAK_HBM_GBS:.1f}% of 8 TB/s)", flush=True)
+              f"{res[name]['gbs']:.0f} GB/s ({100*res[name]['gbs']/PEAK_HBM_GBS:.1f}% of 8 TB/s)  "
+              f"shape {res[name]['shape']}", flush=True)
 
     main_r = res["c32"]
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
@@ -145,7 +161,8 @@ def main():
                      "frac": round(main_r["gbs"] / PEAK_HBM_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,
         "fp64": {"ms_per_matvec": round(res["c64"]["ms"], 4), "GB/s": round(res["c64"]["gbs"], 2),
-                 "frac": round(res["c64"]["gbs"] / PEAK_HBM_GBS, 4)},
+                 "frac": round(res["c64"]["gbs"] / PEAK_HBM_GBS, 4), "launch_shape": res["c64"]["shape"]},
+        "launch_shape": res["c32"]["shape"],
         "setup_s": {"catalog": round(t_cat, 3), "sector": round(t_sec, 3), "total": round(time.time() - t_setup, 3)},
     }
     print(json.dumps(out), flush=True)

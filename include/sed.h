@@ -116,6 +116,16 @@ int sed_matvec_device(sed_sector* sec, const void* x_dev, void* y_dev, int preci
 /* Time n_calls device matvecs on resident random vectors; returns the mean
  * kernel time in ms measured with HIP events on the context's stream. */
 int sed_matvec_benchmark(sed_sector* sec, int precision, int n_calls, double* ms_per_call);
+/* Launch shape of the H|psi> kernel (no reference counterpart; the reference has one
+ * threaded loop).  Shapes are indexed 0..n_shapes-1; each has `lanes` lanes per row
+ * (64/lanes rows per wavefront) and `batch` hops in flight per lane.  shape = -1
+ * re-enables the autotuner, which times every shape on the first matvec of a sector
+ * of dimension >= 2^17 and keeps the fastest (smaller sectors use shape 1). */
+int sed_sector_set_shape(sed_sector* sec, int precision, int shape);
+/* Chosen shape (-1 if not chosen yet), its lanes/batch, the number of shapes, and the
+ * autotuner's mean time per shape in ms (n_shapes floats, 0 if not timed; may be NULL). */
+int sed_sector_shape(const sed_sector* sec, int precision, int* shape, int* lanes, int* batch, int* n_shapes,
+                     float* tune_ms);
 /* diagonalize_block_matrixfree(H_op, n; nev) (:998-1030): lowest nev eigenvalues
  * (ascending) and optionally eigenvectors (sector_dim x nev column-major ComplexF64,
  * may be NULL).  Sectors of dim <= 500 are solved densely, as the reference does;

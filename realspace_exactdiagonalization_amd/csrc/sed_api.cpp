@@ -26,6 +26,7 @@ int launch_density(sed_sector* sec, const double2* vec, double* density);
 int launch_convert(const double* a, float* b, int64_t n, hipStream_t s);
 int launch_convert(const float* a, double* b, int64_t n, hipStream_t s);
 int launch_matvec_kernel_only(sed_sector* sec, const void* xhat, const void* x, void* y, bool fp64, hipStream_t s);
+int matvec_shape_info(const sed_sector* sec, int precision, int* lanes, int* batch, float* ms_per_shape);
 
 template <typename T>
 static int upload(DevBuf<T>& b, const T* h, size_t n, hipStream_t s) {
@@ -552,6 +553,25 @@ int sed_matvec_benchmark(sed_sector* sec, int precision, int n_calls, double* ms
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms_per_call = ms / n_calls;
+    return SED_OK;
+}
+
+int sed_sector_set_shape(sed_sector* sec, int precision, int shape) {
+    if (!sec) return fail(SED_EINVAL, "NULL sector");
+    if (precision != SED_PREC_FP64 && precision != SED_PREC_FP32) return fail(SED_EINVAL, "bad precision");
+    const int n = matvec_shape_info(sec, precision, nullptr, nullptr, nullptr);
+    if (shape < -1 || shape >= n) return fail(SED_EINVAL, "shape out of range");
+    sec->mv_shape[precision == SED_PREC_FP64 ? 0 : 1] = shape;
+    return SED_OK;
+}
+
+int sed_sector_shape(const sed_sector* sec, int precision, int* shape, int* lanes, int* batch, int* n_shapes,
+                     float* tune_ms) {
+    if (!sec) return fail(SED_EINVAL, "NULL sector");
+    if (precision != SED_PREC_FP64 && precision != SED_PREC_FP32) return fail(SED_EINVAL, "bad precision");
+    const int n = matvec_shape_info(sec, precision, lanes, batch, tune_ms);
+    if (shape) *shape = sec->mv_shape[precision == SED_PREC_FP64 ? 0 : 1];
+    if (n_shapes) *n_shapes = n;
     return SED_OK;
 }
 

@@ -113,6 +113,8 @@ struct sed_sector {
     sed::DevBuf<float> d_term_coef32, d_dens_v32, d_chi32;
     sed::DevBuf<double> d_gauge;      // F(rep_r) per row, complex interleaved (only if has_gauge)
     int64_t hops_gated = -1, hops_in_sector = -1;
+    int mv_shape[2] = {-1, -1};       // H|psi> launch shape per precision (FP64, FP32), -1 = not tuned
+    float tune_ms[2][8] = {};         // autotuner timings per shape
     int busy = 0;                     // a server request is running on this sector
     // last solve
     std::vector<double> last_evals;

@@ -29,6 +29,10 @@
 // applies D where vectors cross the boundary.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "sed_device.hpp"
 #include "sed_internal.hpp"
@@ -121,7 +125,11 @@ __device__ __forceinline__ MvLds<R> stage_mv(const MvConst& c) {
     return L;
 }
 
-template <typename M, typename R, bool FERMI>
+// LPR lanes share one row: lane `sub` of the row takes occupied sites sub, sub+LPR,
+// ... (in ascending order), and the row sum is closed by a fixed xor butterfly.  Hops
+// are processed BATCH at a time: BATCH independent table reads are issued before the
+// BATCH dependent gathers, so each lane keeps several memory requests in flight.
+template <typename M, typename R, bool FERMI, int LPR, int BATCH>
 __global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* __restrict__ rep,
                                                      const uint8_t* __restrict__ stab,
                                                      const uint32_t* __restrict__ table,
@@ -129,20 +137,23 @@ __global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* 
                                                      const typename C2<R>::T* __restrict__ x,
                                                      typename C2<R>::T* __restrict__ y, int64_t nblocks) {
     using T = typename C2<R>::T;
+    constexpr int RPB = kMvBlock / LPR;
     const MvLds<R> L = stage_mv<R>(c);
     const int64_t lb = xcd_block(nblocks);
     if (lb < 0) return;
-    const int64_t r = lb * kMvBlock + threadIdx.x;
-    if (r >= c.dim) return;
-    const M m = (M)rep[r];
+    const int sub = threadIdx.x % LPR;
+    const int64_t r = lb * RPB + threadIdx.x / LPR;
+    const bool live = r < c.dim;
+    const M m = live ? (M)rep[r] : (M)0;
     uint32_t* S = L.S + threadIdx.x;
     R ax = 0, ay = 0;   // off-diagonal sum
     R dx = 0, dy = 0;   // diagonal (density) sum
     M rest = m;
-    int ia = 0;
+    for (int i = 0; i < sub && rest; ++i) rest &= rest - 1;
+    int ia = sub;
     while (rest) {
         const int a = ctz_m<M>(rest);
-        rest &= rest - 1;
+        for (int i = 0; i < LPR && rest; ++i) rest &= rest - 1;
         const M mp = m ^ ((M)1 << a);
         {
             uint32_t s0 = 0;
@@ -172,31 +183,58 @@ __global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* 
                 dy += L.dv[d].y;
             }
         }
-        const int t1 = L.tstart[a + 1];
-        for (int t = L.tstart[a]; t < t1; ++t) {
-            const int b = L.tto[t];
-            if ((m >> b) & 1) continue;
-            const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
-            const uint32_t rank = S[ib * kMvBlock] + L.bin[b * c.bs + ib + 1];
-            const uint32_t e = table[rank];
-            if (e == kNoRow) continue;
-            const uint32_t col = e >> c.shift;
-            const int g = (e >> 1) & c.gmask;
-            int p = e & 1;
-            if (FERMI) p ^= (ib - ia) & 1;
-            const T cf = cmul(L.tcoef[t], L.chi[g]);
-            const T v = xhat[col];
-            const T pr = cmul(cf, v);
-            if (p) {
-                ax -= pr.x;
-                ay -= pr.y;
-            } else {
-                ax += pr.x;
-                ay += pr.y;
+        const int t0 = L.tstart[a], t1 = L.tstart[a + 1];
+        for (int t = t0; t < t1; t += BATCH) {
+            uint32_t e[BATCH];
+            int pj[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                e[u] = kNoRow;
+                pj[u] = 0;
+                const int tt = t + u;
+                if (tt < t1) {
+                    const int b = L.tto[tt];
+                    if (!((m >> b) & 1)) {
+                        const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
+                        e[u] = table[S[ib * kMvBlock] + L.bin[b * c.bs + ib + 1]];
+                        if (FERMI) pj[u] = (ib - ia) & 1;
+                    }
+                }
+            }
+            T v[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                v[u].x = 0;
+                v[u].y = 0;
+                if (e[u] != kNoRow) v[u] = xhat[e[u] >> c.shift];
+            }
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                if (e[u] == kNoRow) continue;
+                const int g = (e[u] >> 1) & c.gmask;
+                const T cf = cmul(L.tcoef[t + u], L.chi[g]);
+                const T pr = cmul(cf, v[u]);
+                if (((e[u] & 1) ^ pj[u]) != 0) {
+                    ax -= pr.x;
+                    ay -= pr.y;
+                } else {
+                    ax += pr.x;
+                    ay += pr.y;
+                }
             }
         }
-        ++ia;
+        ia += LPR;
     }
+    if (LPR > 1) {
+#pragma unroll
+        for (int off = LPR / 2; off >= 1; off >>= 1) {
+            ax += __shfl_xor(ax, off, kWave);
+            ay += __shfl_xor(ay, off, kWave);
+            dx += __shfl_xor(dx, off, kWave);
+            dy += __shfl_xor(dy, off, kWave);
+        }
+    }
+    if (!live || sub != 0) return;
     const R inv = (R)1 / sqrt((R)stab[r]);
     const T xr = x[r];
     T out;
@@ -204,6 +242,14 @@ __global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* 
     out.y = ay * inv + (dx * xr.y + dy * xr.x);
     y[r] = out;
 }
+
+// Launch shapes the autotuner chooses from: lanes per row (rows per wavefront =
+// 64 / lanes) and hops per batch.
+struct MvShape {
+    int lanes, batch;
+};
+static const MvShape kShapes[] = {{1, 1}, {1, 4}, {2, 4}, {4, 4}, {8, 4}, {4, 2}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 // Dense H' (row-major, dim x dim, complex double) for the small-sector path.
 template <typename M, bool FERMI>
@@ -380,18 +426,114 @@ size_t matvec_lds_bytes(const sed_sector* sec) {
 template size_t matvec_lds_bytes<float>(const sed_sector*);
 template size_t matvec_lds_bytes<double>(const sed_sector*);
 
-template <typename M, typename R, bool F>
-static int run_mv(sed_sector* sec, const typename C2<R>::T* xh, const typename C2<R>::T* x,
-                  typename C2<R>::T* y, hipStream_t s) {
-    MvConst c = make_const<R>(sec);
-    size_t off[9];
-    const size_t lds = mv_lds_layout<R>(c, off);
-    const int64_t nb = (sec->dim + kMvBlock - 1) / kMvBlock;
+template <typename M, typename R, bool F, int LPR, int B>
+static int launch_shape(const MvConst& c, size_t lds, sed_sector* sec, const typename C2<R>::T* xh,
+                        const typename C2<R>::T* x, typename C2<R>::T* y, hipStream_t s) {
+    const int64_t rpb = kMvBlock / LPR;
+    const int64_t nb = (sec->dim + rpb - 1) / rpb;
     const int64_t grid = ((nb + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_matvec<M, R, F>), dim3((unsigned)grid), dim3(kMvBlock), lds, s, c, sec->d_rep.p,
+    hipLaunchKernelGGL((k_matvec<M, R, F, LPR, B>), dim3((unsigned)grid), dim3(kMvBlock), lds, s, c, sec->d_rep.p,
                        sec->d_stab.p, sec->d_table.p, xh, x, y, nb);
     SED_HIP(hipGetLastError());
     return SED_OK;
+}
+
+template <typename M, typename R, bool F>
+static int run_shape(sed_sector* sec, int shape, const typename C2<R>::T* xh, const typename C2<R>::T* x,
+                     typename C2<R>::T* y, hipStream_t s) {
+    MvConst c = make_const<R>(sec);
+    size_t off[9];
+    const size_t lds = mv_lds_layout<R>(c, off);
+    switch (shape) {
+        case 0: return launch_shape<M, R, F, 1, 1>(c, lds, sec, xh, x, y, s);
+        case 1: return launch_shape<M, R, F, 1, 4>(c, lds, sec, xh, x, y, s);
+        case 2: return launch_shape<M, R, F, 2, 4>(c, lds, sec, xh, x, y, s);
+        case 3: return launch_shape<M, R, F, 4, 4>(c, lds, sec, xh, x, y, s);
+        case 4: return launch_shape<M, R, F, 8, 4>(c, lds, sec, xh, x, y, s);
+        default: return launch_shape<M, R, F, 4, 2>(c, lds, sec, xh, x, y, s);
+    }
+}
+
+template <typename R>
+static int run_any_shape(sed_sector* sec, int shape, const typename C2<R>::T* xh, const typename C2<R>::T* x,
+                         typename C2<R>::T* y, hipStream_t s) {
+    const bool F = sec->cat->stats == SED_FERMIONIC;
+    if (sec->cat->n_site <= 32)
+        return F ? run_shape<uint32_t, R, true>(sec, shape, xh, x, y, s)
+                 : run_shape<uint32_t, R, false>(sec, shape, xh, x, y, s);
+    return F ? run_shape<uint64_t, R, true>(sec, shape, xh, x, y, s)
+             : run_shape<uint64_t, R, false>(sec, shape, xh, x, y, s);
+}
+
+// Autotuner: on the first matvec of a sector large enough for the launch shape to
+// matter, time every shape on the call's own buffers (deterministic kernels, so
+// the repeated launches leave y as one launch would) and keep the fastest.  A
+// process-wide cache keyed by the sector's shape parameters skips re-tuning.
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int, int64_t, int>, int> g_tune_cache;
+constexpr int64_t kTuneMinDim = 1 << 17;
+
+template <typename R>
+static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, const typename C2<R>::T* x,
+                      typename C2<R>::T* y, hipStream_t s) {
+    const int pi = sizeof(R) == 8 ? 0 : 1;
+    if (sec->mv_shape[pi] >= 0) return sec->mv_shape[pi];
+    if (const char* env = std::getenv("SED_MATVEC_SHAPE")) {
+        const int v = std::atoi(env);
+        if (v >= 0 && v < kNumShapes) return sec->mv_shape[pi] = v;
+    }
+    if (sec->dim < kTuneMinDim) return sec->mv_shape[pi] = 1;
+    const auto key = std::make_tuple(sec->cat->n_site, sec->cat->n_filled, sec->cat->n_group, sec->n_terms,
+                                     sec->dim, pi);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) return sec->mv_shape[pi] = it->second;
+    }
+    hipEvent_t e0, e1;
+    SED_HIP(hipEventCreate(&e0));
+    SED_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_i = 1;
+    for (int i = 0; i < kNumShapes; ++i) {
+        SED_TRY(run_any_shape<R>(sec, i, xh, x, y, s));  // warm-up
+        SED_HIP(hipEventRecord(e0, s));
+        for (int k = 0; k < 3; ++k) SED_TRY(run_any_shape<R>(sec, i, xh, x, y, s));
+        SED_HIP(hipEventRecord(e1, s));
+        SED_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SED_HIP(hipEventElapsedTime(&ms, e0, e1));
+        sec->tune_ms[pi][i] = ms / 3;
+        if (ms < best) {
+            best = ms;
+            best_i = i;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = best_i;
+    }
+    return sec->mv_shape[pi] = best_i;
+}
+
+template <typename M, typename R, bool F>
+static int run_mv(sed_sector* sec, const typename C2<R>::T* xh, const typename C2<R>::T* x,
+                  typename C2<R>::T* y, hipStream_t s) {
+    const int shape = pick_shape<R>(sec, xh, x, y, s);
+    if (shape < 0) return shape;
+    return run_shape<M, R, F>(sec, shape, xh, x, y, s);
+}
+
+int matvec_shape_info(const sed_sector* sec, int precision, int* lanes, int* batch, float* ms_per_shape) {
+    const int pi = precision == SED_PREC_FP64 ? 0 : 1;
+    const int sh = sec->mv_shape[pi];
+    if (lanes) *lanes = sh >= 0 ? kShapes[sh].lanes : 0;
+    if (batch) *batch = sh >= 0 ? kShapes[sh].batch : 0;
+    if (ms_per_shape)
+        for (int i = 0; i < kNumShapes; ++i) ms_per_shape[i] = sec->tune_ms[pi][i];
+    return kNumShapes;
 }
 
 template <typename R>

@@ -175,6 +175,17 @@ class SymmetrySectorBasis:
                                          np.zeros(2)))
         self._terms_set = True
 
+    def set_shape(self, precision: int, shape: int):
+        """Force H|psi> launch shape `shape` (-1: autotune)."""
+        check(lib().sed_sector_set_shape(self._h, precision, shape))
+
+    def shape(self, precision: int) -> dict:
+        sh, ln, bt, ns = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        ms = np.zeros(16, dtype=np.float32)
+        check(lib().sed_sector_shape(self._h, precision, C.byref(sh), C.byref(ln), C.byref(bt), C.byref(ns), ptr(ms)))
+        return {"shape": sh.value, "lanes_per_row": ln.value, "batch": bt.value,
+                "tune_ms": [round(float(v), 4) for v in ms[:ns.value]]}
+
     def hop_counts(self) -> Tuple[int, int]:
         g, s = C.c_int64(), C.c_int64()
         check(lib().sed_sector_hop_counts(self._h, C.byref(g), C.byref(s)))

@@ -222,3 +222,29 @@ def test_server_results_alone_and_in_mix(ed, ctx):
         st, ev, stats, lat = srv.result(rid)
         assert st == 0 and np.abs(ev - ref).max() <= 1e-10 * np.abs(ref).max()
     srv.close()
+
+
+def test_every_launch_shape_matches_oracle(ed, ctx, oracle_mod):
+    """Each H|psi> launch shape (lanes per row x hops per batch) gives the oracle's matvec,
+    bitwise identical between two runs of the same shape."""
+    m, G = make("checker3x4_f")
+    ir = L.translation_irreps(G, m.lattice)[5]
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, True)
+    os_ = oracle_mod.Sector(oc, ir.values)
+    os_.set_terms(*L.term_arrays(m))
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    gb = ed.build_symmetry_sector_basis(gc, ir)
+    H = ed.MatrixFreeHamiltonian(gb, m.bilinear_terms, m.density_terms)
+    x = np.random.default_rng(11).standard_normal(gb.dim) + 1j * np.random.default_rng(12).standard_normal(gb.dim)
+    ref = os_.apply_table(x, 1)
+    n_shapes = len(gb.shape(ed.PREC_FP64)["tune_ms"])
+    assert n_shapes >= 4
+    for prec, tol in ((ed.PREC_FP64, 1e-12), (ed.PREC_FP32, 5e-6)):
+        for sh in range(n_shapes):
+            print(f"[shape] prec={prec} shape={sh}", flush=True)
+            gb.set_shape(prec, sh)
+            H.precision = prec
+            y1, y2 = H(x), H(x)
+            assert np.array_equal(y1, y2)
+            assert np.abs(y1 - ref).max() <= tol * np.abs(ref).max()
