@@ -75,6 +75,68 @@ def cpu_baseline(model, G, ir, cat, basis, seconds, threads, bytes_c64, gated, i
             "s_per_matvec": round(t_full, 3)}
 
 
+def flux_scan(E, L, ctx, n_twists=16, nev=3, lanes=8):
+    """Config[3]-style flux scan: Haldane-boson 3x4 (24 sites, 6 bosons), all 12 momentum
+    sectors x n_twists twists theta_x in [0, 1), lowest nev eigenvalues each.
+
+    Three ways: the plain FP64 path one solve after another (sed_solve), and the
+    continuous-batching server with FP64 and with FP32+FP64 refinement.  Returns
+    solves/s, p50/p99 latency and the largest eigenvalue difference between the modes."""
+    import numpy as np
+    base = L.workload("haldane3x4")
+    keep, sectors = [], []
+    for i in range(n_twists):
+        th = (i / n_twists, 0.0)
+        m = L.twisted(base, th)
+        G = L.model_group(m, th)
+        cat = E.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                             particle_statistics=m.statistics, ctx=ctx)
+        keep.append(cat)
+        for ir in L.translation_irreps(G, m.lattice):
+            b = E.build_symmetry_sector_basis(cat, ir)
+            b.set_terms(m.bilinear_terms, m.density_terms)
+            sectors.append(b)
+    out = {"solves": len(sectors), "dims": [min(b.dim for b in sectors), max(b.dim for b in sectors)], "nev": nev}
+    # plain FP64, sequential
+    ev_plain, lat = [], []
+    t0 = time.time()
+    for b in sectors:
+        H = E.MatrixFreeHamiltonian.__new__(E.MatrixFreeHamiltonian)
+        H.basis, H.n, H.precision = b, b.dim, E.PREC_FP64
+        t1 = time.time()
+        ev_plain.append(E.diagonalize_block_matrixfree(H, b.dim, nev=nev, return_vectors=False)[0])
+        lat.append(time.time() - t1)
+    wall = time.time() - t0
+    out["plain_fp64"] = {"solves_per_s": round(len(sectors) / wall, 2), "wall_s": round(wall, 3),
+                         "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
+                         "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2)}
+    for name, prec in (("batched_fp64", E.PREC_FP64), ("batched_fp32_refine", E.PREC_FP32_REFINE)):
+        srv = E.SectorServer(ctx, max_slots=lanes)
+        t0 = time.time()
+        ids = [srv.submit(b, nev, precision=prec) for b in sectors]
+        srv.run(0)
+        wall = time.time() - t0
+        lat, err, iters = [], 0.0, []
+        for rid, ref in zip(ids, ev_plain):
+            st, ev, stats, l = srv.result(rid)
+            if st != 0:
+                raise RuntimeError(f"request {rid} failed with status {st}")
+            lat.append(l)
+            err = max(err, float(np.max(np.abs(ev - ref) / np.maximum(np.abs(ref), 1e-300))))
+            iters.append(stats["iters_fp32"] + stats["iters_fp64"])
+        srv.close()
+        out[name] = {"solves_per_s": round(len(sectors) / wall, 2), "wall_s": round(wall, 3),
+                     "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
+                     "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
+                     "max_rel_diff_vs_plain_fp64": err, "mean_lanczos_steps": round(float(np.mean(iters)), 1),
+                     "lanes": lanes}
+    out["speedup_batched_fp32_refine_vs_plain_fp64"] = round(
+        out["batched_fp32_refine"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"], 3)
+    for b in sectors:
+        b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +145,8 @@ def main():
     ap.add_argument("--workload", default="checker4x6")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-flux", action="store_true", help="skip the batched flux-scan solves")
+    ap.add_argument("--lanes", type=int, default=8, help="server lanes for the flux scan")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -137,6 +201,13 @@ def main():
               f"{res[name]['gbs']:.0f} GB/s ({100*res[name]['gbs']/PEAK_HBM_GBS:.1f}% of 8 TB/s)  "
               f"shape {res[name]['shape']}", flush=True)
 
+    flux = None
+    if not args.no_flux:
+        flux = flux_scan(E, L, ctx, lanes=args.lanes)
+        print(f"[bench] flux scan {flux['solves']} solves: plain fp64 {flux['plain_fp64']['solves_per_s']}/s, "
+              f"batched fp64 {flux['batched_fp64']['solves_per_s']}/s, batched fp32+refine "
+              f"{flux['batched_fp32_refine']['solves_per_s']}/s (p99 {flux['batched_fp32_refine']['p99_ms']} ms)",
+              flush=True)
     main_r = res["c32"]
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     cpu = None if args.no_cpu else cpu_baseline(model, G, ir, cat, basis, args.cpu_seconds, threads,
@@ -163,6 +234,7 @@ def main():
         "fp64": {"ms_per_matvec": round(res["c64"]["ms"], 4), "GB/s": round(res["c64"]["gbs"], 2),
                  "frac": round(res["c64"]["gbs"] / PEAK_HBM_GBS, 4), "launch_shape": res["c64"]["shape"]},
         "launch_shape": res["c32"]["shape"],
+        "flux_scan_haldane3x4": flux,
         "setup_s": {"catalog": round(t_cat, 3), "sector": round(t_sec, 3), "total": round(time.time() - t_setup, 3)},
     }
     print(json.dumps(out), flush=True)

@@ -25,7 +25,7 @@ size_t matvec_lds_bytes(const sed_sector* sec);
 int launch_density(sed_sector* sec, const double2* vec, double* density);
 int launch_convert(const double* a, float* b, int64_t n, hipStream_t s);
 int launch_convert(const float* a, double* b, int64_t n, hipStream_t s);
-int launch_matvec_kernel_only(sed_sector* sec, const void* xhat, const void* x, void* y, bool fp64, hipStream_t s);
+int launch_matvec_xhat(sed_sector* sec, const void* xhat, void* y, bool fp64, hipStream_t s);
 int matvec_shape_info(const sed_sector* sec, int precision, int* lanes, int* batch, float* ms_per_shape);
 
 template <typename T>
@@ -437,6 +437,7 @@ int sed_sector_set_terms(sed_sector* sec, int n_bil, const int32_t* i_from, cons
         return fail(SED_EUNSUPPORTED, "term lists need " + std::to_string(need) + " bytes of LDS per workgroup");
     sec->has_terms = true;
     sec->hops_gated = sec->hops_in_sector = -1;
+    ++sec->version;
     return SED_OK;
 }
 
@@ -540,12 +541,12 @@ int sed_matvec_benchmark(sed_sector* sec, int precision, int n_calls, double* ms
     // one full call (prescale + kernel) fills xhat; timed calls run the H|psi> kernel alone
     if (fp64) SED_TRY(launch_matvec64(sec, (const double*)x.p, (double*)y.p, (double*)xh.p, s));
     else SED_TRY(launch_matvec32(sec, (const float*)x.p, (float*)y.p, (float*)xh.p, s));
-    for (int i = 0; i < 2; ++i) SED_TRY(launch_matvec_kernel_only(sec, xh.p, x.p, y.p, fp64, s));
+    for (int i = 0; i < 2; ++i) SED_TRY(launch_matvec_xhat(sec, xh.p, y.p, fp64, s));
     hipEvent_t e0, e1;
     SED_HIP(hipEventCreate(&e0));
     SED_HIP(hipEventCreate(&e1));
     SED_HIP(hipEventRecord(e0, s));
-    for (int i = 0; i < n_calls; ++i) SED_TRY(launch_matvec_kernel_only(sec, xh.p, x.p, y.p, fp64, s));
+    for (int i = 0; i < n_calls; ++i) SED_TRY(launch_matvec_xhat(sec, xh.p, y.p, fp64, s));
     SED_HIP(hipEventRecord(e1, s));
     SED_HIP(hipEventSynchronize(e1));
     float ms = 0;

@@ -114,6 +114,7 @@ struct sed_sector {
     sed::DevBuf<double> d_gauge;      // F(rep_r) per row, complex interleaved (only if has_gauge)
     int64_t hops_gated = -1, hops_in_sector = -1;
     int mv_shape[2] = {-1, -1};       // H|psi> launch shape per precision (FP64, FP32), -1 = not tuned
+    int64_t version = 0;              // bumped whenever device term tables are (re)uploaded
     float tune_ms[2][8] = {};         // autotuner timings per shape
     int busy = 0;                     // a server request is running on this sector
     // last solve
@@ -126,14 +127,34 @@ namespace sed {
 
 // Reusable device/pinned buffers of one solver lane (one stream).  Grown on demand,
 // never shrunk, so repeated solves do not allocate in the hot loop.
+// Everything a captured Lanczos step depends on: the sector and its term upload
+// version, the precision, sizes, launch shape and every device address in the graph.
+struct GraphKey {
+    const void* sec = nullptr;
+    int64_t version = -1;
+    int prec = -1, m = 0, shape = -1;
+    int64_t n = 0;
+    const void* ptrs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool operator==(const GraphKey& o) const {
+        if (sec != o.sec || version != o.version || prec != o.prec || m != o.m || shape != o.shape || n != o.n)
+            return false;
+        for (int i = 0; i < 5; ++i)
+            if (ptrs[i] != o.ptrs[i]) return false;
+        return true;
+    }
+};
 struct Workspace {
     hipStream_t stream = nullptr;
     DevBuf<double> V64, w64, xh64;    // complex interleaved doubles
     DevBuf<float> V32, w32, xh32;     // complex interleaved floats
     DevBuf<double> part, coef, S, v0;
-    double* hcoef = nullptr;          // pinned host copy of projection coefficients
+    DevBuf<double> state;             // Lanczos coefficients, alpha/beta, step counter
+    double* hcoef = nullptr;          // pinned host copy of alpha/beta and flags
     size_t hcoef_n = 0;
+    hipGraphExec_t graph_exec = nullptr;  // one captured Lanczos step
+    GraphKey graph_key;
     int ensure_host(size_t n_doubles);
+    void drop_graph();
     void release();
 };
 template <typename T>

@@ -8,13 +8,19 @@
 //     HBM; every new vector is orthogonalised twice against the whole basis
 //     (classical Gram-Schmidt, "twice is enough"), and the alpha/beta recurrence and
 //     every projection coefficient are carried in FP64 whatever the vector precision;
+//   * one Lanczos step is a fixed sequence of kernels whose step index j lives in device
+//     memory: coefficients, norms, alpha_j and beta_j never leave the device inside a
+//     step, so steps run back to back without host synchronisation and the sequence is
+//     captured once into a hipGraph and replayed;
 //   * all reductions run over a fixed grid in a fixed order (bitwise reproducible);
-//   * the projected matrix is diagonalised on the host (order <= m);
+//   * the host reads alpha/beta every few steps, diagonalises the projected matrix
+//     (order <= m), checks the residuals and drives the thick restarts;
 //   * SED_PREC_FP32_REFINE runs the loop with complex-FP32 vectors and matvec, then
 //     restarts an FP64 loop from the sum of the FP32 Ritz vectors.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "sed_device.hpp"
@@ -23,11 +29,14 @@
 namespace sed {
 
 int launch_dense(sed_sector* sec, double* H_dev, hipStream_t s);
+int launch_matvec_xhat(sed_sector* sec, const void* xhat, void* y, bool fp64, hipStream_t s);
 
 constexpr int kRedGrid = 512;     // fixed reduction grid (determinism)
 constexpr int kRedBlock = 256;
 constexpr int kDotGroup = 8;
 constexpr int kRitzRows = 32;
+constexpr int kMaxBasis = 64;   // largest Krylov basis (m)
+constexpr int kUpdGrid = 1024;  // fixed grid of the update / norm kernels
 
 __device__ __forceinline__ double2 to_d2(double2 v) { return v; }
 __device__ __forceinline__ double2 to_d2(float2 v) { return make_double2(v.x, v.y); }
@@ -46,17 +55,23 @@ __device__ __forceinline__ double2 wave_sum_d2(double2 v) {
     return v;
 }
 
-// partial[i * gridDim + b] = sum over block b's rows of conj(V_i[r]) w[r], i < nv;
-// partial[nv * gridDim + b].x = sum |w[r]|^2 when with_norm.
+
+// ---- one Lanczos step, step index j = *step on the device ----------------------
+// The step reads v_j through xhat = sqrt(stab) v_j (written by the previous step),
+// w = H v_j, then two classical Gram-Schmidt passes against V[0..j], then
+// v_{j+1} = w' / |w'| and its xhat.
+
+// part[i * G + b] = sum over block b's rows of conj(V_i[r]) w[r] for i <= j, and
+// part[(j+1) * G + b] = sum |w[r]|^2.
 template <typename T>
-__global__ void __launch_bounds__(kRedBlock) k_dots(const T* __restrict__ V, int64_t ld, int nv,
-                                                    const T* __restrict__ w, int64_t n, int with_norm,
-                                                    double2* __restrict__ part) {
+__global__ void __launch_bounds__(kRedBlock) k_dots(const T* __restrict__ V, int64_t n, const int* __restrict__ step,
+                                                    const T* __restrict__ w, double2* __restrict__ part) {
     __shared__ double2 red[kDotGroup][kRedBlock / kWave];
+    const int nv = *step + 1;
+    const int total = nv + 1;
     const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t r0 = (int64_t)blockIdx.x * chunk;
     const int64_t r1 = std::min<int64_t>(n, r0 + chunk);
-    const int total = nv + (with_norm ? 1 : 0);
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     for (int g0 = 0; g0 < total; g0 += kDotGroup) {
         double2 acc[kDotGroup];
@@ -68,37 +83,38 @@ __global__ void __launch_bounds__(kRedBlock) k_dots(const T* __restrict__ V, int
             for (int q = 0; q < kDotGroup; ++q) {
                 const int i = g0 + q;
                 if (i < nv) {
-                    const double2 v = to_d2(V[(int64_t)i * ld + r]);
+                    const double2 v = to_d2(V[(int64_t)i * n + r]);
                     acc[q].x += v.x * wr.x + v.y * wr.y;
                     acc[q].y += v.x * wr.y - v.y * wr.x;
-                } else if (i == nv && with_norm) {
+                } else if (i == nv) {
                     acc[q].x += wr.x * wr.x + wr.y * wr.y;
                 }
             }
         }
 #pragma unroll
         for (int q = 0; q < kDotGroup; ++q) {
-            const double2 s = wave_sum_d2(acc[q]);
-            if (lane == 0) red[q][wv] = s;
+            const double2 sm = wave_sum_d2(acc[q]);
+            if (lane == 0) red[q][wv] = sm;
         }
         __syncthreads();
         if (threadIdx.x < kDotGroup && g0 + (int)threadIdx.x < total) {
-            double2 s = make_double2(0.0, 0.0);
+            double2 sm = make_double2(0.0, 0.0);
             for (int i = 0; i < kRedBlock / kWave; ++i) {
-                s.x += red[threadIdx.x][i].x;
-                s.y += red[threadIdx.x][i].y;
+                sm.x += red[threadIdx.x][i].x;
+                sm.y += red[threadIdx.x][i].y;
             }
-            part[(int64_t)(g0 + threadIdx.x) * gridDim.x + blockIdx.x] = s;
+            part[(int64_t)(g0 + threadIdx.x) * gridDim.x + blockIdx.x] = sm;
         }
         __syncthreads();
     }
 }
 
-// out[i] = sum_b part[i * G + b], fixed order
+// out[i] = sum_b part[i * G + b] for i <= j + 1 (fixed order); launched with m + 2 blocks.
 __global__ void __launch_bounds__(kRedBlock) k_reduce(const double2* __restrict__ part, int G,
-                                                      double2* __restrict__ out) {
+                                                      const int* __restrict__ step, double2* __restrict__ out) {
     __shared__ double2 red[kRedBlock / kWave];
     const int i = blockIdx.x;
+    if (i > *step + 1) return;
     double2 acc = make_double2(0.0, 0.0);
     for (int b = threadIdx.x; b < G; b += kRedBlock) {
         acc.x += part[(int64_t)i * G + b].x;
@@ -108,35 +124,130 @@ __global__ void __launch_bounds__(kRedBlock) k_reduce(const double2* __restrict_
     if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double2 s = make_double2(0.0, 0.0);
+        double2 sm = make_double2(0.0, 0.0);
         for (int k = 0; k < kRedBlock / kWave; ++k) {
-            s.x += red[k].x;
-            s.y += red[k].y;
+            sm.x += red[k].x;
+            sm.y += red[k].y;
         }
-        out[i] = s;
+        out[i] = sm;
     }
 }
 
-// out[r] = (w[r] - sum_{i<nv} V_i[r] h[i]) * scale ; out may alias w
+// out[r] = w[r] - sum_{i<=j} V_i[r] h[i]; out = w (to_next = 0) or V_{j+1} (to_next = 1).
+// With to_next, each block also writes the partial |out|^2 of its rows (fixed grid).
 template <typename T>
-__global__ void __launch_bounds__(256) k_update(const T* __restrict__ V, int64_t ld, int nv,
-                                                const double2* __restrict__ h, const T* w, int64_t n,
-                                                T* out, double scale) {
-    extern __shared__ double2 lh[];
+__global__ void __launch_bounds__(256) k_update(T* V, int64_t n, const int* __restrict__ step,
+                                                const double2* __restrict__ h, T* w, int to_next,
+                                                double* __restrict__ normpart) {
+    __shared__ double2 lh[kMaxBasis + 2];
+    __shared__ double red[256 / kWave];
+    const int nv = *step + 1;
     for (int i = threadIdx.x; i < nv; i += blockDim.x) lh[i] = h[i];
     __syncthreads();
+    T* out = to_next ? V + (int64_t)nv * n : w;
+    double nrm = 0.0;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
         double2 a = to_d2(w[r]);
         for (int i = 0; i < nv; ++i) {
-            const double2 v = to_d2(V[(int64_t)i * ld + r]);
+            const double2 v = to_d2(V[(int64_t)i * n + r]);
             const double2 c = lh[i];
             a.x -= v.x * c.x - v.y * c.y;
             a.y -= v.x * c.y + v.y * c.x;
         }
-        a.x *= scale;
-        a.y *= scale;
-        out[r] = from_d2<T>(a);
+        const T o = from_d2<T>(a);
+        out[r] = o;
+        const double2 od = to_d2(o);
+        nrm += od.x * od.x + od.y * od.y;
     }
+    if (!to_next) return;
+    for (int off = 32; off > 0; off >>= 1) nrm += __shfl_down(nrm, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sm = 0.0;
+        for (int k = 0; k < 256 / kWave; ++k) sm += red[k];
+        normpart[blockIdx.x] = sm;
+    }
+}
+
+// Single block: |v_{j+1}|^2 from the update's partials, alpha_j, beta_j, 1/beta_j, and
+// the step counter advances to j + 1.  A vanishing residual (invariant subspace) is
+// flagged with the step at which it happened and scales the new vector by zero.
+__global__ void __launch_bounds__(kRedBlock) k_finalize(int* step, const double2* __restrict__ h1,
+                                                        const double2* __restrict__ h2,
+                                                        const double* __restrict__ normpart, int G,
+                                                        double* __restrict__ alpha, double* __restrict__ beta,
+                                                        double* __restrict__ scale, int* __restrict__ flag) {
+    __shared__ double red[kRedBlock / kWave];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < G; b += kRedBlock) acc += normpart[b];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double nrm2 = 0.0;
+    for (int k = 0; k < kRedBlock / kWave; ++k) nrm2 += red[k];
+    const int j = *step;
+    const double a = h1[j].x + h2[j].x;
+    const double hv = h1[j + 1].x;   // |H v_j|^2 before orthogonalisation
+    const double b = sqrt(nrm2);
+    const bool brk = !(b > 1e-12 * sqrt(hv)) || !(b > 0.0);
+    alpha[j] = a;
+    beta[j] = b;
+    *scale = brk ? 0.0 : 1.0 / b;
+    if (brk && *flag == 0) *flag = j + 1;
+    *step = j + 1;
+}
+
+// v_j *= scale and xhat = sqrt(stab) v_j for j = *step (the vector the next step applies H to).
+template <typename T>
+__global__ void __launch_bounds__(256) k_scale(T* V, int64_t n, const int* __restrict__ step,
+                                               const double* __restrict__ scale, const uint8_t* __restrict__ stab,
+                                               T* __restrict__ xhat) {
+    T* v = V + (int64_t)(*step) * n;
+    const double sc = *scale;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const double2 a = to_d2(v[r]);
+        const T o = from_d2<T>(make_double2(a.x * sc, a.y * sc));
+        v[r] = o;
+        const double2 od = to_d2(o);
+        const double s = sqrt((double)stab[r]);
+        xhat[r] = from_d2<T>(make_double2(od.x * s, od.y * s));
+    }
+}
+
+// |v_0|^2 partials for the start vector (fixed grid), then k_start_finalize
+template <typename T>
+__global__ void __launch_bounds__(256) k_normsq(const T* __restrict__ v, int64_t n, double* __restrict__ normpart) {
+    __shared__ double red[256 / kWave];
+    double nrm = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const double2 a = to_d2(v[r]);
+        nrm += a.x * a.x + a.y * a.y;
+    }
+    for (int off = 32; off > 0; off >>= 1) nrm += __shfl_down(nrm, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sm = 0.0;
+        for (int k = 0; k < 256 / kWave; ++k) sm += red[k];
+        normpart[blockIdx.x] = sm;
+    }
+}
+
+__global__ void __launch_bounds__(kRedBlock) k_start_finalize(int* step, int start, const double* __restrict__ normpart,
+                                                              int G, double* __restrict__ scale) {
+    __shared__ double red[kRedBlock / kWave];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < G; b += kRedBlock) acc += normpart[b];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double nrm2 = 0.0;
+    for (int k = 0; k < kRedBlock / kWave; ++k) nrm2 += red[k];
+    *scale = nrm2 > 0.0 ? 1.0 / sqrt(nrm2) : 0.0;
+    *step = start;
 }
 
 // In place, row by row: V_i[r] <- sum_{j<m} V_j[r] S[j * kk + i] for i < kk (S real).
@@ -227,10 +338,17 @@ int Workspace::ensure_host(size_t n_doubles) {
     return SED_OK;
 }
 
+void Workspace::drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+    graph_key = GraphKey();
+}
+
 void Workspace::release() {
+    drop_graph();
     V64.release(); w64.release(); xh64.release();
     V32.release(); w32.release(); xh32.release();
-    part.release(); coef.release(); S.release(); v0.release();
+    part.release(); coef.release(); S.release(); v0.release(); state.release();
     if (hcoef) (void)hipHostFree(hcoef);
     hcoef = nullptr;
     hcoef_n = 0;
@@ -240,32 +358,28 @@ template <typename T>
 struct VecTraits;
 template <>
 struct VecTraits<double2> {
+    static constexpr bool fp64 = true;
     static double* V(Workspace& w) { return w.V64.p; }
     static double* W(Workspace& w) { return w.w64.p; }
+    static double* X(Workspace& w) { return w.xh64.p; }
     static int reserve(Workspace& w, int64_t n, int m) {
         SED_TRY(grow(w.V64, (size_t)2 * (m + 1) * n));
         SED_TRY(grow(w.w64, (size_t)2 * n));
         SED_TRY(grow(w.xh64, (size_t)2 * n));
         return SED_OK;
     }
-    static int matvec(sed_sector* s, const double2* x, double2* y, Workspace& w) {
-        return launch_matvec64(s, reinterpret_cast<const double*>(x), reinterpret_cast<double*>(y), w.xh64.p,
-                               w.stream);
-    }
 };
 template <>
 struct VecTraits<float2> {
+    static constexpr bool fp64 = false;
     static float* V(Workspace& w) { return w.V32.p; }
     static float* W(Workspace& w) { return w.w32.p; }
+    static float* X(Workspace& w) { return w.xh32.p; }
     static int reserve(Workspace& w, int64_t n, int m) {
         SED_TRY(grow(w.V32, (size_t)2 * (m + 1) * n));
         SED_TRY(grow(w.w32, (size_t)2 * n));
         SED_TRY(grow(w.xh32, (size_t)2 * n));
         return SED_OK;
-    }
-    static int matvec(sed_sector* s, const float2* x, float2* y, Workspace& w) {
-        return launch_matvec32(s, reinterpret_cast<const float*>(x), reinterpret_cast<float*>(y), w.xh32.p,
-                               w.stream);
     }
 };
 
@@ -278,6 +392,13 @@ struct SolveOut {
     int64_t matvecs = 0;
 };
 
+static bool graphs_enabled() {
+    const char* e = std::getenv("SED_LANCZOS_GRAPH");
+    return !(e && e[0] == '0');
+}
+
+// Device state block (doubles): h1 (2*(m+2)), h2 (2*(m+2)), alpha (m), beta (m), scale (1);
+// ints after it: step, flag.
 template <typename T>
 class ThickLanczos {
    public:
@@ -285,14 +406,28 @@ class ThickLanczos {
 
     int init() {
         SED_TRY(VecTraits<T>::reserve(ws_, n_, m_));
-        SED_TRY(grow(ws_.part, (size_t)2 * (m_ + 2) * kRedGrid));
-        SED_TRY(grow(ws_.coef, (size_t)2 * (m_ + 2)));
+        G_ = (int)std::max<int64_t>(1, std::min<int64_t>(kRedGrid, (n_ + kRedBlock - 1) / kRedBlock));
+        SED_TRY(grow(ws_.part, (size_t)2 * (m_ + 2) * G_ + kUpdGrid));
         SED_TRY(grow(ws_.S, (size_t)(m_ + 1) * (m_ + 1)));
-        SED_TRY(ws_.ensure_host((size_t)2 * (m_ + 2)));
+        const size_t nd = 4 * (size_t)(m_ + 2) + 2 * (size_t)m_ + 1;
+        SED_TRY(grow(ws_.state, nd + 4));
+        SED_TRY(ws_.ensure_host(nd + 8));
         V_ = reinterpret_cast<T*>(VecTraits<T>::V(ws_));
         w_ = reinterpret_cast<T*>(VecTraits<T>::W(ws_));
+        xh_ = reinterpret_cast<T*>(VecTraits<T>::X(ws_));
         part_ = reinterpret_cast<double2*>(ws_.part.p);
-        coef_ = reinterpret_cast<double2*>(ws_.coef.p);
+        normpart_ = ws_.part.p + (size_t)2 * (m_ + 2) * G_;
+        double* st = ws_.state.p;
+        h1_ = reinterpret_cast<double2*>(st);
+        h2_ = reinterpret_cast<double2*>(st + 2 * (m_ + 2));
+        alpha_ = st + 4 * (m_ + 2);
+        beta_ = alpha_ + m_;
+        scale_ = beta_ + m_;
+        step_ = reinterpret_cast<int*>(scale_ + 1);
+        flag_ = step_ + 1;
+        int zero[2] = {0, 0};
+        SED_HIP(hipMemcpyAsync(step_, zero, sizeof(zero), hipMemcpyHostToDevice, st_));
+        SED_HIP(hipStreamSynchronize(st_));  // `zero` is a stack temporary
         return SED_OK;
     }
     T* vec(int i) { return V_ + (size_t)i * n_; }
@@ -300,88 +435,75 @@ class ThickLanczos {
     int start_random(uint64_t seed) {
         hipLaunchKernelGGL(k_random<T>, dim3(grid_for(n_)), dim3(256), 0, st_, vec(0), n_, seed);
         SED_HIP(hipGetLastError());
-        return normalize0();
+        return normalize_start(0);
     }
     int start_from(const double2* v0) {
         hipLaunchKernelGGL((k_convert<double2, T>), dim3(grid_for(n_)), dim3(256), 0, st_, v0, vec(0), n_);
         SED_HIP(hipGetLastError());
-        return normalize0();
+        return normalize_start(0);
     }
 
     int run(int nev, double tol, int maxiter, SolveOut& out) {
         std::vector<double> T_((size_t)m_ * m_, 0.0);  // row-major projected matrix
-        std::vector<double2> h1(m_ + 2), h2(m_ + 2);
         int k = 0;
         double beta = 0.0;
         out = SolveOut();
         std::vector<double> theta, Sm;
         int jm = 0;
         bool done = false, breakdown = false;
+        // first matvec eagerly: it fixes the launch shape (autotuner) before any capture
+        const bool use_graph = graphs_enabled();
         while (!done) {
-            for (int j = k; j < m_; ++j) {
-                SED_TRY(VecTraits<T>::matvec(sec_, vec(j), w_, ws_));
-                ++out.matvecs;
-                ++out.iters;
-                SED_TRY(dots(j + 1, w_, false, h1.data()));
-                SED_TRY(update(j + 1, w_, w_, 1.0));
-                SED_TRY(dots(j + 1, w_, true, h2.data()));
-                const double nrm2 = h2[j + 1].x;
-                double p2 = 0.0;
-                for (int i = 0; i <= j; ++i) p2 += h2[i].x * h2[i].x + h2[i].y * h2[i].y;
-                double b2 = nrm2 - p2;
-                T_[(size_t)j * m_ + j] = h1[j].x + h2[j].x;
-                if (!(b2 > 0.5 * nrm2)) {
-                    // heavy cancellation: apply the second correction and measure directly
-                    SED_TRY(update(j + 1, w_, w_, 1.0));
-                    std::vector<double2> h3(2);
-                    SED_TRY(dots(0, w_, true, h3.data()));
-                    b2 = h3[0].x;
-                    std::fill(h2.begin(), h2.end(), make_double2(0.0, 0.0));
-                    SED_TRY(upload_coef(j + 1, h2.data()));
-                }
-                beta = std::sqrt(std::max(b2, 0.0));
-                double tnorm = 0.0;
-                for (int i = 0; i <= j; ++i) tnorm = std::max(tnorm, std::fabs(T_[(size_t)i * m_ + i]));
-                jm = j + 1;
-                if (beta <= 1e-13 * std::max(1.0, tnorm)) {
+            int j = k;
+            while (j < m_) {
+                int chunk = std::min(m_ - j, j < nev + 2 ? nev + 2 - j : kCheckEvery);
+                chunk = std::max(1, std::min(chunk, maxiter - out.iters));
+                for (int c = 0; c < chunk; ++c) SED_TRY(step(use_graph && out.iters > 0));
+                out.iters += chunk;
+                out.matvecs += chunk;
+                j += chunk;
+                // read alpha/beta/flag of this cycle
+                const size_t na = (size_t)m_;
+                SED_HIP(hipMemcpyAsync(ws_.hcoef, alpha_, (2 * na + 1) * sizeof(double), hipMemcpyDeviceToHost, st_));
+                SED_HIP(hipMemcpyAsync(ws_.hcoef + 2 * na + 2, step_, 2 * sizeof(int), hipMemcpyDeviceToHost, st_));
+                SED_HIP(hipStreamSynchronize(st_));
+                const double* ha = ws_.hcoef;
+                const double* hb = ws_.hcoef + na;
+                int flag = 0;
+                std::memcpy(&flag, reinterpret_cast<const char*>(ws_.hcoef + 2 * na + 2) + sizeof(int), sizeof(int));
+                jm = j;
+                if (flag > 0 && flag <= j) {
                     breakdown = true;
+                    jm = flag;   // v_flag could not be formed: basis V[0..flag-1]
+                }
+                for (int i = k; i < jm; ++i) {
+                    T_[(size_t)i * m_ + i] = ha[i];
+                    if (i + 1 < jm) {
+                        T_[(size_t)(i + 1) * m_ + i] = hb[i];
+                        T_[(size_t)i * m_ + i + 1] = hb[i];
+                    }
+                }
+                beta = breakdown ? 0.0 : hb[jm - 1];
+                eig(jm, T_, theta, Sm);
+                int conv = 0;
+                for (int i = 0; i < std::min(nev, jm); ++i) {
+                    const double res = beta * std::fabs(Sm[(size_t)(jm - 1) * jm + i]);
+                    if (res <= tol) ++conv; else break;
+                }
+                if (breakdown || conv >= nev || out.iters >= maxiter) {
+                    done = true;
                     break;
                 }
-                // coef_ still holds h2: v_{j+1} = (w - V h2) / beta
-                SED_TRY(update(j + 1, w_, vec(j + 1), 1.0 / beta));
-                if (j + 1 < m_) {
-                    T_[(size_t)(j + 1) * m_ + j] = beta;
-                    T_[(size_t)j * m_ + j + 1] = beta;
-                }
-                const bool check = (j + 1 == m_) || ((j + 1 - k) % 8 == 0 && j + 1 >= nev + 2) ||
-                                   out.iters >= maxiter;
-                if (check) {
-                    eig(jm, T_, theta, Sm);
-                    int conv = 0;
-                    for (int i = 0; i < std::min(nev, jm); ++i) {
-                        const double res = beta * std::fabs(Sm[(size_t)(jm - 1) * jm + i]);
-                        if (res <= tol) ++conv; else break;
-                    }
-                    if (conv >= nev || out.iters >= maxiter) {
-                        done = true;
-                        break;
-                    }
-                }
-            }
-            if (breakdown) {
-                eig(jm, T_, theta, Sm);
-                beta = 0.0;
-                done = true;
             }
             if (done) break;
-            // thick restart: kk Ritz vectors + the residual direction
-            eig(jm, T_, theta, Sm);
+            // thick restart: kk Ritz vectors + the residual direction v_m
             const int kk = std::min(m_ - 2, nev + std::max(4, (m_ - nev) / 3));
             std::vector<double> Sk((size_t)jm * kk);
             for (int jj = 0; jj < jm; ++jj)
                 for (int i = 0; i < kk; ++i) Sk[(size_t)jj * kk + i] = Sm[(size_t)jj * jm + i];
             SED_TRY(ritz(jm, Sk, kk));
             SED_HIP(hipMemcpyAsync(vec(kk), vec(jm), (size_t)n_ * sizeof(T), hipMemcpyDeviceToDevice, st_));
+            SED_TRY(set_step_and_xhat(kk));
             std::fill(T_.begin(), T_.end(), 0.0);
             for (int i = 0; i < kk; ++i) {
                 T_[(size_t)i * m_ + i] = theta[i];
@@ -413,38 +535,83 @@ class ThickLanczos {
     T* V_ = nullptr;
 
    private:
-    int normalize0() {
-        std::vector<double2> h(2);
-        SED_TRY(dots(0, vec(0), true, h.data()));
-        const double nrm = std::sqrt(h[0].x);
-        if (!(nrm > 0)) return fail(SED_EINVAL, "zero start vector");
-        hipLaunchKernelGGL(k_update<T>, dim3(grid_for(n_)), dim3(256), 16, st_, vec(0), (int64_t)n_, 0, coef_,
-                           vec(0), (int64_t)n_, vec(0), 1.0 / nrm);
+    static constexpr int kCheckEvery = 8;
+
+    // one Lanczos step as plain launches
+    int launch_step() {
+        SED_TRY(launch_matvec_xhat(sec_, xh_, w_, VecTraits<T>::fp64, st_));
+        hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_);
+        hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h1_);
+        hipLaunchKernelGGL(k_update<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h1_, w_, 0,
+                           normpart_);
+        hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_);
+        hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h2_);
+        hipLaunchKernelGGL(k_update<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h2_, w_, 1,
+                           normpart_);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kRedBlock), 0, st_, step_, h1_, h2_, normpart_, kUpdGrid, alpha_,
+                           beta_, scale_, flag_);
+        hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n_)), dim3(256), 0, st_, V_, (int64_t)n_, step_, scale_,
+                           sec_->d_stab.p, xh_);
         SED_HIP(hipGetLastError());
         return SED_OK;
     }
-    int dots(int nv, const T* w, bool with_norm, double2* host_out) {
-        const int total = nv + (with_norm ? 1 : 0);
-        hipLaunchKernelGGL(k_dots<T>, dim3(kRedGrid), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, nv, w, (int64_t)n_,
-                           with_norm ? 1 : 0, part_);
-        SED_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_reduce, dim3(total), dim3(kRedBlock), 0, st_, part_, kRedGrid, coef_);
-        SED_HIP(hipGetLastError());
-        SED_HIP(hipMemcpyAsync(ws_.hcoef, coef_, total * sizeof(double2), hipMemcpyDeviceToHost, st_));
-        SED_HIP(hipStreamSynchronize(st_));
-        std::memcpy(host_out, ws_.hcoef, total * sizeof(double2));
+
+    // a step through the workspace's captured graph (captured on first use for this
+    // sector / precision / basis size)
+    int step(bool graph) {
+        if (!graph) return launch_step();
+        GraphKey key;
+        key.sec = sec_;
+        key.version = sec_->version;
+        key.prec = VecTraits<T>::fp64 ? 0 : 1;
+        key.m = m_;
+        key.shape = sec_->mv_shape[key.prec];
+        key.n = n_;
+        key.ptrs[0] = V_;
+        key.ptrs[1] = w_;
+        key.ptrs[2] = xh_;
+        key.ptrs[3] = part_;
+        key.ptrs[4] = ws_.state.p;
+        if (!ws_.graph_exec || !(ws_.graph_key == key)) {
+            ws_.drop_graph();
+            hipGraph_t g = nullptr;
+            SED_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+            const int rc = launch_step();
+            hipError_t e = hipStreamEndCapture(st_, &g);
+            if (rc != SED_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(SED_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&ws_.graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                ws_.graph_exec = nullptr;
+                return fail(SED_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            }
+            ws_.graph_key = key;
+        }
+        SED_HIP(hipGraphLaunch(ws_.graph_exec, st_));
         return SED_OK;
     }
-    int upload_coef(int nv, const double2* h) {
-        std::memcpy(ws_.hcoef, h, nv * sizeof(double2));
-        SED_HIP(hipMemcpyAsync(coef_, ws_.hcoef, nv * sizeof(double2), hipMemcpyHostToDevice, st_));
+
+    int normalize_start(int j) {
+        hipLaunchKernelGGL(k_normsq<T>, dim3(kUpdGrid), dim3(256), 0, st_, vec(j), (int64_t)n_, normpart_);
+        hipLaunchKernelGGL(k_start_finalize, dim3(1), dim3(kRedBlock), 0, st_, step_, j, normpart_, kUpdGrid, scale_);
+        hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n_)), dim3(256), 0, st_, V_, (int64_t)n_, step_, scale_,
+                           sec_->d_stab.p, xh_);
+        SED_HIP(hipGetLastError());
         return SED_OK;
     }
-    // out <- (w - V coef_) * scale, coef_ already on device
-    int update(int nv, const T* w, T* out, double scale) {
-        hipLaunchKernelGGL(k_update<T>, dim3(grid_for(n_)), dim3(256), nv * sizeof(double2), st_, V_, (int64_t)n_,
-                           nv, coef_, w, (int64_t)n_, out, scale);
+    // after a restart: step = j and xhat from v_j (v_j already unit norm)
+    int set_step_and_xhat(int j) {
+        const double one = 1.0;
+        SED_HIP(hipMemcpyAsync(step_, &j, sizeof(int), hipMemcpyHostToDevice, st_));
+        SED_HIP(hipMemcpyAsync(scale_, &one, sizeof(double), hipMemcpyHostToDevice, st_));
+        hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n_)), dim3(256), 0, st_, V_, (int64_t)n_, step_, scale_,
+                           sec_->d_stab.p, xh_);
         SED_HIP(hipGetLastError());
+        SED_HIP(hipStreamSynchronize(st_));  // the host values above are stack temporaries
         return SED_OK;
     }
     int ritz(int m, const std::vector<double>& S, int kk) {
@@ -454,8 +621,7 @@ class ThickLanczos {
         hipLaunchKernelGGL(k_ritz<T>, dim3(g), dim3(kRitzRows), lds, st_, V_, (int64_t)n_, m, ws_.S.p, kk,
                            (int64_t)n_);
         SED_HIP(hipGetLastError());
-        // S is host-pageable: make sure the copy has been consumed before it goes out of scope
-        SED_HIP(hipStreamSynchronize(st_));
+        SED_HIP(hipStreamSynchronize(st_));  // S is host-pageable
         return SED_OK;
     }
     static void eig(int jm, const std::vector<double>& T_, std::vector<double>& theta, std::vector<double>& S) {
@@ -471,14 +637,19 @@ class ThickLanczos {
     int m_;
     Workspace& ws_;
     hipStream_t st_;
+    int G_ = 1;
     T* w_ = nullptr;
+    T* xh_ = nullptr;
     double2* part_ = nullptr;
-    double2* coef_ = nullptr;
+    double* normpart_ = nullptr;
+    double2 *h1_ = nullptr, *h2_ = nullptr;
+    double *alpha_ = nullptr, *beta_ = nullptr, *scale_ = nullptr;
+    int *step_ = nullptr, *flag_ = nullptr;
 };
 
 static int krylov_dim(int64_t n, int nev) {
     int m = std::max(2 * nev + 24, 40);
-    return (int)std::min<int64_t>(m, std::min<int64_t>(n - 1, 64));
+    return (int)std::min<int64_t>(m, std::min<int64_t>(n - 1, kMaxBasis));
 }
 
 int64_t solve_footprint(const sed_sector* sec, int nev, int precision) {

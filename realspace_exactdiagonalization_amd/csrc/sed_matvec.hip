@@ -134,7 +134,6 @@ __global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* 
                                                      const uint8_t* __restrict__ stab,
                                                      const uint32_t* __restrict__ table,
                                                      const typename C2<R>::T* __restrict__ xhat,
-                                                     const typename C2<R>::T* __restrict__ x,
                                                      typename C2<R>::T* __restrict__ y, int64_t nblocks) {
     using T = typename C2<R>::T;
     constexpr int RPB = kMvBlock / LPR;
@@ -235,11 +234,12 @@ __global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* 
         }
     }
     if (!live || sub != 0) return;
+    // diag(m_r) x[r] = diag(m_r) xhat[r] / sqrt(stab_r): one scale for both parts
     const R inv = (R)1 / sqrt((R)stab[r]);
-    const T xr = x[r];
+    const T xr = xhat[r];
     T out;
-    out.x = ax * inv + (dx * xr.x - dy * xr.y);
-    out.y = ay * inv + (dx * xr.y + dy * xr.x);
+    out.x = (ax + (dx * xr.x - dy * xr.y)) * inv;
+    out.y = (ay + (dx * xr.y + dy * xr.x)) * inv;
     y[r] = out;
 }
 
@@ -428,41 +428,39 @@ template size_t matvec_lds_bytes<double>(const sed_sector*);
 
 template <typename M, typename R, bool F, int LPR, int B>
 static int launch_shape(const MvConst& c, size_t lds, sed_sector* sec, const typename C2<R>::T* xh,
-                        const typename C2<R>::T* x, typename C2<R>::T* y, hipStream_t s) {
+                        typename C2<R>::T* y, hipStream_t s) {
     const int64_t rpb = kMvBlock / LPR;
     const int64_t nb = (sec->dim + rpb - 1) / rpb;
     const int64_t grid = ((nb + 7) / 8) * 8;
     hipLaunchKernelGGL((k_matvec<M, R, F, LPR, B>), dim3((unsigned)grid), dim3(kMvBlock), lds, s, c, sec->d_rep.p,
-                       sec->d_stab.p, sec->d_table.p, xh, x, y, nb);
+                       sec->d_stab.p, sec->d_table.p, xh, y, nb);
     SED_HIP(hipGetLastError());
     return SED_OK;
 }
 
 template <typename M, typename R, bool F>
-static int run_shape(sed_sector* sec, int shape, const typename C2<R>::T* xh, const typename C2<R>::T* x,
-                     typename C2<R>::T* y, hipStream_t s) {
+static int run_shape(sed_sector* sec, int shape, const typename C2<R>::T* xh, typename C2<R>::T* y,
+                     hipStream_t s) {
     MvConst c = make_const<R>(sec);
     size_t off[9];
     const size_t lds = mv_lds_layout<R>(c, off);
     switch (shape) {
-        case 0: return launch_shape<M, R, F, 1, 1>(c, lds, sec, xh, x, y, s);
-        case 1: return launch_shape<M, R, F, 1, 4>(c, lds, sec, xh, x, y, s);
-        case 2: return launch_shape<M, R, F, 2, 4>(c, lds, sec, xh, x, y, s);
-        case 3: return launch_shape<M, R, F, 4, 4>(c, lds, sec, xh, x, y, s);
-        case 4: return launch_shape<M, R, F, 8, 4>(c, lds, sec, xh, x, y, s);
-        default: return launch_shape<M, R, F, 4, 2>(c, lds, sec, xh, x, y, s);
+        case 0: return launch_shape<M, R, F, 1, 1>(c, lds, sec, xh, y, s);
+        case 1: return launch_shape<M, R, F, 1, 4>(c, lds, sec, xh, y, s);
+        case 2: return launch_shape<M, R, F, 2, 4>(c, lds, sec, xh, y, s);
+        case 3: return launch_shape<M, R, F, 4, 4>(c, lds, sec, xh, y, s);
+        case 4: return launch_shape<M, R, F, 8, 4>(c, lds, sec, xh, y, s);
+        default: return launch_shape<M, R, F, 4, 2>(c, lds, sec, xh, y, s);
     }
 }
 
 template <typename R>
-static int run_any_shape(sed_sector* sec, int shape, const typename C2<R>::T* xh, const typename C2<R>::T* x,
-                         typename C2<R>::T* y, hipStream_t s) {
+static int run_any_shape(sed_sector* sec, int shape, const typename C2<R>::T* xh, typename C2<R>::T* y,
+                         hipStream_t s) {
     const bool F = sec->cat->stats == SED_FERMIONIC;
     if (sec->cat->n_site <= 32)
-        return F ? run_shape<uint32_t, R, true>(sec, shape, xh, x, y, s)
-                 : run_shape<uint32_t, R, false>(sec, shape, xh, x, y, s);
-    return F ? run_shape<uint64_t, R, true>(sec, shape, xh, x, y, s)
-             : run_shape<uint64_t, R, false>(sec, shape, xh, x, y, s);
+        return F ? run_shape<uint32_t, R, true>(sec, shape, xh, y, s) : run_shape<uint32_t, R, false>(sec, shape, xh, y, s);
+    return F ? run_shape<uint64_t, R, true>(sec, shape, xh, y, s) : run_shape<uint64_t, R, false>(sec, shape, xh, y, s);
 }
 
 // Autotuner: on the first matvec of a sector large enough for the launch shape to
@@ -474,8 +472,7 @@ static std::map<std::tuple<int, int, int, int, int64_t, int>, int> g_tune_cache;
 constexpr int64_t kTuneMinDim = 1 << 17;
 
 template <typename R>
-static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, const typename C2<R>::T* x,
-                      typename C2<R>::T* y, hipStream_t s) {
+static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<R>::T* y, hipStream_t s) {
     const int pi = sizeof(R) == 8 ? 0 : 1;
     if (sec->mv_shape[pi] >= 0) return sec->mv_shape[pi];
     if (const char* env = std::getenv("SED_MATVEC_SHAPE")) {
@@ -496,9 +493,9 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, const typena
     float best = 1e30f;
     int best_i = 1;
     for (int i = 0; i < kNumShapes; ++i) {
-        SED_TRY(run_any_shape<R>(sec, i, xh, x, y, s));  // warm-up
+        SED_TRY(run_any_shape<R>(sec, i, xh, y, s));  // warm-up
         SED_HIP(hipEventRecord(e0, s));
-        for (int k = 0; k < 3; ++k) SED_TRY(run_any_shape<R>(sec, i, xh, x, y, s));
+        for (int k = 0; k < 3; ++k) SED_TRY(run_any_shape<R>(sec, i, xh, y, s));
         SED_HIP(hipEventRecord(e1, s));
         SED_HIP(hipEventSynchronize(e1));
         float ms = 0;
@@ -518,12 +515,11 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, const typena
     return sec->mv_shape[pi] = best_i;
 }
 
-template <typename M, typename R, bool F>
-static int run_mv(sed_sector* sec, const typename C2<R>::T* xh, const typename C2<R>::T* x,
-                  typename C2<R>::T* y, hipStream_t s) {
-    const int shape = pick_shape<R>(sec, xh, x, y, s);
+template <typename R>
+static int run_mv(sed_sector* sec, const typename C2<R>::T* xh, typename C2<R>::T* y, hipStream_t s) {
+    const int shape = pick_shape<R>(sec, xh, y, s);
     if (shape < 0) return shape;
-    return run_shape<M, R, F>(sec, shape, xh, x, y, s);
+    return run_any_shape<R>(sec, shape, xh, y, s);
 }
 
 int matvec_shape_info(const sed_sector* sec, int precision, int* lanes, int* batch, float* ms_per_shape) {
@@ -545,10 +541,7 @@ static int matvec_any(sed_sector* sec, const typename C2<R>::T* x, typename C2<R
     const int64_t g = std::min<int64_t>((sec->dim + 255) / 256, 1 << 20);
     hipLaunchKernelGGL(k_prescale<T>, dim3((unsigned)g), dim3(256), 0, s, x, sec->d_stab.p, xh, sec->dim);
     SED_HIP(hipGetLastError());
-    const bool F = sec->cat->stats == SED_FERMIONIC;
-    if (sec->cat->n_site <= 32)
-        return F ? run_mv<uint32_t, R, true>(sec, xh, x, y, s) : run_mv<uint32_t, R, false>(sec, xh, x, y, s);
-    return F ? run_mv<uint64_t, R, true>(sec, xh, x, y, s) : run_mv<uint64_t, R, false>(sec, xh, x, y, s);
+    return run_mv<R>(sec, xh, y, s);
 }
 
 int launch_matvec64(sed_sector* sec, const double* x, double* y, double* xhat, hipStream_t s) {
@@ -560,22 +553,11 @@ int launch_matvec32(sed_sector* sec, const float* x, float* y, float* xhat, hipS
                              reinterpret_cast<float2*>(xhat), s);
 }
 
-// Main kernel only (the prescaled vector xhat already in place): for timing.
-int launch_matvec_kernel_only(sed_sector* sec, const void* xhat, const void* x, void* y, bool fp64, hipStream_t s) {
-    const bool F = sec->cat->stats == SED_FERMIONIC;
-    const bool small = sec->cat->n_site <= 32;
-    if (fp64) {
-        auto xh = reinterpret_cast<const double2*>(xhat);
-        auto xx = reinterpret_cast<const double2*>(x);
-        auto yy = reinterpret_cast<double2*>(y);
-        if (small) return F ? run_mv<uint32_t, double, true>(sec, xh, xx, yy, s) : run_mv<uint32_t, double, false>(sec, xh, xx, yy, s);
-        return F ? run_mv<uint64_t, double, true>(sec, xh, xx, yy, s) : run_mv<uint64_t, double, false>(sec, xh, xx, yy, s);
-    }
-    auto xh = reinterpret_cast<const float2*>(xhat);
-    auto xx = reinterpret_cast<const float2*>(x);
-    auto yy = reinterpret_cast<float2*>(y);
-    if (small) return F ? run_mv<uint32_t, float, true>(sec, xh, xx, yy, s) : run_mv<uint32_t, float, false>(sec, xh, xx, yy, s);
-    return F ? run_mv<uint64_t, float, true>(sec, xh, xx, yy, s) : run_mv<uint64_t, float, false>(sec, xh, xx, yy, s);
+// H|psi> kernel alone on an already prescaled xhat = sqrt(stab) x (Lanczos steps, timing).
+int launch_matvec_xhat(sed_sector* sec, const void* xhat, void* y, bool fp64, hipStream_t s) {
+    if (sec->dim == 0) return SED_OK;
+    if (fp64) return run_mv<double>(sec, reinterpret_cast<const double2*>(xhat), reinterpret_cast<double2*>(y), s);
+    return run_mv<float>(sec, reinterpret_cast<const float2*>(xhat), reinterpret_cast<float2*>(y), s);
 }
 
 int launch_gauge64(sed_sector* sec, double* v, bool forward, hipStream_t s) {

@@ -248,3 +248,25 @@ def test_every_launch_shape_matches_oracle(ed, ctx, oracle_mod):
             y1, y2 = H(x), H(x)
             assert np.array_equal(y1, y2)
             assert np.abs(y1 - ref).max() <= tol * np.abs(ref).max()
+
+
+def test_lanczos_graph_replay_matches_eager(ed, ctx, monkeypatch):
+    """The captured Lanczos step (hipGraph) gives bitwise the eigenvalues of eager launches,
+    over several solves in a row that reuse the graph with eager work in between."""
+    m, G = make("checker3x4_f")
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    irs = L.translation_irreps(G, m.lattice)
+    res = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("SED_LANCZOS_GRAPH", mode)
+        out = []
+        for ir in (irs[0], irs[3], irs[0]):
+            gb = ed.build_symmetry_sector_basis(gc, ir)
+            H, n = ed.hamiltonian_linear_operator(gb, m.bilinear_terms, m.density_terms)
+            for prec in (ed.PREC_FP64, ed.PREC_FP32_REFINE):
+                out.append(ed.diagonalize_block_matrixfree(H, n, nev=3, precision=prec, return_vectors=False)[0])
+                H(np.ones(n, dtype=np.complex128))   # eager work between replays
+        res[mode] = out
+    for a, b in zip(res["0"], res["1"]):
+        assert np.array_equal(a, b)
