@@ -181,6 +181,27 @@ int solve_sector(sed_sector* sec, int nev, double tol, uint64_t seed, int maxite
 // device bytes a solve of this sector needs in a workspace
 int64_t solve_footprint(const sed_sector* sec, int nev, int precision);
 
+// ---- packed Lanczos engine: many small sectors, one workgroup each, per launch ----
+struct PackJob {
+    int64_t id = 0;
+    sed_sector* sec = nullptr;
+    int nev = 1, maxiter = 300, precision = SED_PREC_FP64;
+    double tol = 1e-10;
+    uint64_t seed = 0;
+    int status = 1;
+    std::string error;
+    std::vector<double> evals;
+    sed_solve_stats stats{};
+};
+class PackEngine;
+// on_done is called from the engine thread, once per job, after its results are set.
+PackEngine* pack_create(sed_context* ctx, int64_t arena_bytes, int max_active, void (*on_done)(void*, PackJob*),
+                        void* user);
+void pack_submit(PackEngine* e, PackJob* job);
+void pack_destroy(PackEngine* e);
+bool pack_eligible(const sed_sector* sec, int nev);
+int64_t pack_footprint(const sed_sector* sec, int nev, int precision);
+
 // ---- host eigensolvers (sed_eigen.cpp) ----
 // Real symmetric n x n (row-major, overwritten) -> ascending eigenvalues, eigenvectors in columns.
 void jacobi_eigh(int n, std::vector<double>& a, std::vector<double>& evals, std::vector<double>& evecs);

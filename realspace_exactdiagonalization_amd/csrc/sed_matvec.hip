@@ -36,99 +36,14 @@
 
 #include "sed_device.hpp"
 #include "sed_internal.hpp"
+#include "sed_mv.hpp"
 
 namespace sed {
 
-constexpr int kMvBlock = 256;
-
-struct MvConst {
-    const uint32_t* bin;
-    const int32_t* tstart;
-    const uint8_t* tto;
-    const void* tcoef;
-    const int32_t* dstart;
-    const uint8_t* dj;
-    const void* dv;
-    const void* chi;
-    int n, k, nG, bs, shift, n_terms, n_dens;
-    uint32_t gmask;
-    int64_t dim;
-};
-
-template <typename R>
-struct MvLds {
-    using T = typename C2<R>::T;
-    uint32_t* bin;
-    int32_t* tstart;
-    int32_t* dstart;
-    T* tcoef;
-    T* dv;
-    T* chi;
-    uint8_t* tto;
-    uint8_t* dj;
-    uint32_t* S;
-};
-
-__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
-
-template <typename R>
-__host__ __device__ inline size_t mv_lds_layout(const MvConst& c, size_t* off) {
-    using T = typename C2<R>::T;
-    size_t o = 0;
-    off[0] = o; o = align16(o + (size_t)(c.n + 1) * c.bs * 4);       // bin
-    off[1] = o; o = align16(o + (size_t)(c.n + 1) * 4);               // tstart
-    off[2] = o; o = align16(o + (size_t)(c.n + 1) * 4);               // dstart
-    off[3] = o; o = align16(o + (size_t)c.n_terms * sizeof(T));       // tcoef
-    off[4] = o; o = align16(o + (size_t)c.n_dens * sizeof(T));        // dv
-    off[5] = o; o = align16(o + (size_t)c.nG * sizeof(T));            // chi
-    off[6] = o; o = align16(o + (size_t)c.n_terms);                   // tto
-    off[7] = o; o = align16(o + (size_t)c.n_dens);                    // dj
-    off[8] = o; o = align16(o + (size_t)(c.k > 0 ? c.k : 1) * kMvBlock * 4);  // S
-    return o;
-}
-
-template <typename R>
-__device__ __forceinline__ MvLds<R> stage_mv(const MvConst& c) {
-    using T = typename C2<R>::T;
-    extern __shared__ __align__(16) uint8_t smem[];
-    size_t off[9];
-    mv_lds_layout<R>(c, off);
-    MvLds<R> L;
-    L.bin = reinterpret_cast<uint32_t*>(smem + off[0]);
-    L.tstart = reinterpret_cast<int32_t*>(smem + off[1]);
-    L.dstart = reinterpret_cast<int32_t*>(smem + off[2]);
-    L.tcoef = reinterpret_cast<T*>(smem + off[3]);
-    L.dv = reinterpret_cast<T*>(smem + off[4]);
-    L.chi = reinterpret_cast<T*>(smem + off[5]);
-    L.tto = smem + off[6];
-    L.dj = smem + off[7];
-    L.S = reinterpret_cast<uint32_t*>(smem + off[8]);
-    const int nt = blockDim.x;
-    for (int i = threadIdx.x; i < (c.n + 1) * c.bs; i += nt) L.bin[i] = c.bin[i];
-    for (int i = threadIdx.x; i <= c.n; i += nt) {
-        L.tstart[i] = c.tstart[i];
-        L.dstart[i] = c.dstart[i];
-    }
-    const T* tc = reinterpret_cast<const T*>(c.tcoef);
-    const T* dv = reinterpret_cast<const T*>(c.dv);
-    const T* ch = reinterpret_cast<const T*>(c.chi);
-    for (int i = threadIdx.x; i < c.n_terms; i += nt) {
-        L.tcoef[i] = tc[i];
-        L.tto[i] = c.tto[i];
-    }
-    for (int i = threadIdx.x; i < c.n_dens; i += nt) {
-        L.dv[i] = dv[i];
-        L.dj[i] = c.dj[i];
-    }
-    for (int i = threadIdx.x; i < c.nG; i += nt) L.chi[i] = ch[i];
-    __syncthreads();
-    return L;
-}
-
-// LPR lanes share one row: lane `sub` of the row takes occupied sites sub, sub+LPR,
-// ... (in ascending order), and the row sum is closed by a fixed xor butterfly.  Hops
-// are processed BATCH at a time: BATCH independent table reads are issued before the
-// BATCH dependent gathers, so each lane keeps several memory requests in flight.
+// LPR lanes share one row (rows per block = kMvBlock / LPR); the row sum is closed by a
+// fixed xor butterfly over the row's lanes.  Hops are processed BATCH at a time: BATCH
+// independent table reads are issued before the BATCH dependent gathers, so each lane
+// keeps several memory requests in flight.
 template <typename M, typename R, bool FERMI, int LPR, int BATCH>
 __global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* __restrict__ rep,
                                                      const uint8_t* __restrict__ stab,
@@ -137,93 +52,15 @@ __global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* 
                                                      typename C2<R>::T* __restrict__ y, int64_t nblocks) {
     using T = typename C2<R>::T;
     constexpr int RPB = kMvBlock / LPR;
-    const MvLds<R> L = stage_mv<R>(c);
+    const MvLds<R> L = stage_mv<R>(c, kMvBlock);
     const int64_t lb = xcd_block(nblocks);
     if (lb < 0) return;
     const int sub = threadIdx.x % LPR;
     const int64_t r = lb * RPB + threadIdx.x / LPR;
     const bool live = r < c.dim;
     const M m = live ? (M)rep[r] : (M)0;
-    uint32_t* S = L.S + threadIdx.x;
-    R ax = 0, ay = 0;   // off-diagonal sum
-    R dx = 0, dy = 0;   // diagonal (density) sum
-    M rest = m;
-    for (int i = 0; i < sub && rest; ++i) rest &= rest - 1;
-    int ia = sub;
-    while (rest) {
-        const int a = ctz_m<M>(rest);
-        for (int i = 0; i < LPR && rest; ++i) rest &= rest - 1;
-        const M mp = m ^ ((M)1 << a);
-        {
-            uint32_t s0 = 0;
-            M t = mp;
-            int j = 0;
-            while (t) {
-                const int q = ctz_m<M>(t);
-                t &= t - 1;
-                s0 += L.bin[q * c.bs + j + 2];
-                ++j;
-            }
-            S[0] = s0;
-            t = mp;
-            j = 0;
-            uint32_t cur = s0;
-            while (t) {
-                const int q = ctz_m<M>(t);
-                t &= t - 1;
-                cur += L.bin[q * c.bs + j + 1] - L.bin[q * c.bs + j + 2];
-                ++j;
-                S[j * kMvBlock] = cur;
-            }
-        }
-        for (int d = L.dstart[a]; d < L.dstart[a + 1]; ++d) {
-            if ((m >> L.dj[d]) & 1) {
-                dx += L.dv[d].x;
-                dy += L.dv[d].y;
-            }
-        }
-        const int t0 = L.tstart[a], t1 = L.tstart[a + 1];
-        for (int t = t0; t < t1; t += BATCH) {
-            uint32_t e[BATCH];
-            int pj[BATCH];
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
-                e[u] = kNoRow;
-                pj[u] = 0;
-                const int tt = t + u;
-                if (tt < t1) {
-                    const int b = L.tto[tt];
-                    if (!((m >> b) & 1)) {
-                        const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
-                        e[u] = table[S[ib * kMvBlock] + L.bin[b * c.bs + ib + 1]];
-                        if (FERMI) pj[u] = (ib - ia) & 1;
-                    }
-                }
-            }
-            T v[BATCH];
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
-                v[u].x = 0;
-                v[u].y = 0;
-                if (e[u] != kNoRow) v[u] = xhat[e[u] >> c.shift];
-            }
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
-                if (e[u] == kNoRow) continue;
-                const int g = (e[u] >> 1) & c.gmask;
-                const T cf = cmul(L.tcoef[t + u], L.chi[g]);
-                const T pr = cmul(cf, v[u]);
-                if (((e[u] & 1) ^ pj[u]) != 0) {
-                    ax -= pr.x;
-                    ay -= pr.y;
-                } else {
-                    ax += pr.x;
-                    ay += pr.y;
-                }
-            }
-        }
-        ia += LPR;
-    }
+    R ax = 0, ay = 0, dx = 0, dy = 0;
+    row_sums<M, R, FERMI, LPR, BATCH, kMvBlock>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx, dy);
     if (LPR > 1) {
 #pragma unroll
         for (int off = LPR / 2; off >= 1; off >>= 1) {
@@ -257,7 +94,7 @@ __global__ void __launch_bounds__(kMvBlock) k_dense(MvConst c, const uint64_t* _
                                                     const uint8_t* __restrict__ stab,
                                                     const uint32_t* __restrict__ table,
                                                     double2* __restrict__ H) {
-    const MvLds<double> L = stage_mv<double>(c);
+    const MvLds<double> L = stage_mv<double>(c, kMvBlock);
     const int64_t r = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
     if (r >= c.dim) return;
     const M m = (M)rep[r];
@@ -304,7 +141,7 @@ template <typename M>
 __global__ void __launch_bounds__(kMvBlock) k_hop_counts(MvConst c, const uint64_t* __restrict__ rep,
                                                          const uint32_t* __restrict__ table,
                                                          unsigned long long* __restrict__ partial) {
-    const MvLds<double> L = stage_mv<double>(c);
+    const MvLds<double> L = stage_mv<double>(c, kMvBlock);
     __shared__ unsigned long long red[2][kMvBlock / kWave];
     const int64_t r = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
     unsigned long long ng = 0, ns = 0;
@@ -417,11 +254,13 @@ static MvConst make_const(const sed_sector* sec) {
     return c;
 }
 
+MvConst mv_const(const sed_sector* sec, bool fp64) { return fp64 ? make_const<double>(sec) : make_const<float>(sec); }
+
 template <typename R>
 size_t matvec_lds_bytes(const sed_sector* sec) {
     MvConst c = make_const<R>(sec);
     size_t off[9];
-    return mv_lds_layout<R>(c, off);
+    return mv_lds_layout<R>(c, off, kMvBlock);
 }
 template size_t matvec_lds_bytes<float>(const sed_sector*);
 template size_t matvec_lds_bytes<double>(const sed_sector*);
@@ -443,7 +282,7 @@ static int run_shape(sed_sector* sec, int shape, const typename C2<R>::T* xh, ty
                      hipStream_t s) {
     MvConst c = make_const<R>(sec);
     size_t off[9];
-    const size_t lds = mv_lds_layout<R>(c, off);
+    const size_t lds = mv_lds_layout<R>(c, off, kMvBlock);
     switch (shape) {
         case 0: return launch_shape<M, R, F, 1, 1>(c, lds, sec, xh, y, s);
         case 1: return launch_shape<M, R, F, 1, 4>(c, lds, sec, xh, y, s);
@@ -583,7 +422,7 @@ static int run_counts(sed_sector* sec, int64_t* gated, int64_t* ins) {
     hipStream_t s = sec->cat->ctx->stream;
     MvConst c = make_const<double>(sec);
     size_t off[9];
-    const size_t lds = mv_lds_layout<double>(c, off);
+    const size_t lds = mv_lds_layout<double>(c, off, kMvBlock);
     const int64_t nb = std::max<int64_t>(1, (sec->dim + kMvBlock - 1) / kMvBlock);
     DevBuf<unsigned long long> part;
     SED_TRY(part.alloc(2 * nb));
@@ -615,7 +454,7 @@ int launch_hop_counts(sed_sector* sec, int64_t* gated, int64_t* in_sector) {
 int launch_dense(sed_sector* sec, double* H_dev, hipStream_t s) {
     MvConst c = make_const<double>(sec);
     size_t off[9];
-    const size_t lds = mv_lds_layout<double>(c, off);
+    const size_t lds = mv_lds_layout<double>(c, off, kMvBlock);
     SED_HIP(hipMemsetAsync(H_dev, 0, (size_t)sec->dim * sec->dim * 16, s));
     const int64_t nb = (sec->dim + kMvBlock - 1) / kMvBlock;
     const bool F = sec->cat->stats == SED_FERMIONIC;

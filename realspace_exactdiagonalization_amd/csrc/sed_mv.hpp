@@ -1,0 +1,184 @@
+// Shared pieces of the row-gather H|psi> (device constants in LDS and the per-row
+// hop loop), used by the stand-alone matvec and by the packed Lanczos kernel.
+#pragma once
+
+#include "sed_device.hpp"
+
+namespace sed {
+
+constexpr int kMvBlock = 256;
+
+struct MvConst {
+    const uint32_t* bin;
+    const int32_t* tstart;
+    const uint8_t* tto;
+    const void* tcoef;
+    const int32_t* dstart;
+    const uint8_t* dj;
+    const void* dv;
+    const void* chi;
+    int n, k, nG, bs, shift, n_terms, n_dens;
+    uint32_t gmask;
+    int64_t dim;
+};
+
+template <typename R>
+struct MvLds {
+    using T = typename C2<R>::T;
+    uint32_t* bin;
+    int32_t* tstart;
+    int32_t* dstart;
+    T* tcoef;
+    T* dv;
+    T* chi;
+    uint8_t* tto;
+    uint8_t* dj;
+    uint32_t* S;
+};
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+template <typename R>
+__host__ __device__ inline size_t mv_lds_layout(const MvConst& c, size_t* off, int block) {
+    using T = typename C2<R>::T;
+    size_t o = 0;
+    off[0] = o; o = align16(o + (size_t)(c.n + 1) * c.bs * 4);       // bin
+    off[1] = o; o = align16(o + (size_t)(c.n + 1) * 4);               // tstart
+    off[2] = o; o = align16(o + (size_t)(c.n + 1) * 4);               // dstart
+    off[3] = o; o = align16(o + (size_t)c.n_terms * sizeof(T));       // tcoef
+    off[4] = o; o = align16(o + (size_t)c.n_dens * sizeof(T));        // dv
+    off[5] = o; o = align16(o + (size_t)c.nG * sizeof(T));            // chi
+    off[6] = o; o = align16(o + (size_t)c.n_terms);                   // tto
+    off[7] = o; o = align16(o + (size_t)c.n_dens);                    // dj
+    off[8] = o; o = align16(o + (size_t)(c.k > 0 ? c.k : 1) * block * 4);    // S
+    return o;
+}
+
+template <typename R>
+__device__ __forceinline__ MvLds<R> stage_mv(const MvConst& c, int block) {
+    using T = typename C2<R>::T;
+    extern __shared__ __align__(16) uint8_t smem[];
+    size_t off[9];
+    mv_lds_layout<R>(c, off, block);
+    MvLds<R> L;
+    L.bin = reinterpret_cast<uint32_t*>(smem + off[0]);
+    L.tstart = reinterpret_cast<int32_t*>(smem + off[1]);
+    L.dstart = reinterpret_cast<int32_t*>(smem + off[2]);
+    L.tcoef = reinterpret_cast<T*>(smem + off[3]);
+    L.dv = reinterpret_cast<T*>(smem + off[4]);
+    L.chi = reinterpret_cast<T*>(smem + off[5]);
+    L.tto = smem + off[6];
+    L.dj = smem + off[7];
+    L.S = reinterpret_cast<uint32_t*>(smem + off[8]);
+    const int nt = blockDim.x;
+    for (int i = threadIdx.x; i < (c.n + 1) * c.bs; i += nt) L.bin[i] = c.bin[i];
+    for (int i = threadIdx.x; i <= c.n; i += nt) {
+        L.tstart[i] = c.tstart[i];
+        L.dstart[i] = c.dstart[i];
+    }
+    const T* tc = reinterpret_cast<const T*>(c.tcoef);
+    const T* dv = reinterpret_cast<const T*>(c.dv);
+    const T* ch = reinterpret_cast<const T*>(c.chi);
+    for (int i = threadIdx.x; i < c.n_terms; i += nt) {
+        L.tcoef[i] = tc[i];
+        L.tto[i] = c.tto[i];
+    }
+    for (int i = threadIdx.x; i < c.n_dens; i += nt) {
+        L.dv[i] = dv[i];
+        L.dj[i] = c.dj[i];
+    }
+    for (int i = threadIdx.x; i < c.nG; i += nt) L.chi[i] = ch[i];
+    __syncthreads();
+    return L;
+}
+
+// Hop sums of one row m for the lane with index `sub` among the LPR lanes of the row:
+// it takes occupied sites sub, sub+LPR, ... in ascending order.  S points at this
+// thread's rank scratch (k entries, stride SST).  Returns the off-diagonal sum
+// (ax, ay) = sum conj(t') s_JW sign chi(g) xhat[c] and the density sum (dx, dy).
+template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST>
+__device__ __forceinline__ void row_sums(const MvLds<R>& L, const MvConst& c, const M m, const int sub,
+                                         uint32_t* S, const uint32_t* __restrict__ table,
+                                         const typename C2<R>::T* __restrict__ xhat, R& ax, R& ay, R& dx,
+                                         R& dy) {
+    using T = typename C2<R>::T;
+    M rest = m;
+    for (int i = 0; i < sub && rest; ++i) rest &= rest - 1;
+    int ia = sub;
+    while (rest) {
+        const int a = ctz_m<M>(rest);
+        for (int i = 0; i < LPR && rest; ++i) rest &= rest - 1;
+        const M mp = m ^ ((M)1 << a);
+        {
+            uint32_t s0 = 0;
+            M t = mp;
+            int j = 0;
+            while (t) {
+                const int q = ctz_m<M>(t);
+                t &= t - 1;
+                s0 += L.bin[q * c.bs + j + 2];
+                ++j;
+            }
+            S[0] = s0;
+            t = mp;
+            j = 0;
+            uint32_t cur = s0;
+            while (t) {
+                const int q = ctz_m<M>(t);
+                t &= t - 1;
+                cur += L.bin[q * c.bs + j + 1] - L.bin[q * c.bs + j + 2];
+                ++j;
+                S[j * SST] = cur;
+            }
+        }
+        for (int d = L.dstart[a]; d < L.dstart[a + 1]; ++d) {
+            if ((m >> L.dj[d]) & 1) {
+                dx += L.dv[d].x;
+                dy += L.dv[d].y;
+            }
+        }
+        const int t0 = L.tstart[a], t1 = L.tstart[a + 1];
+        for (int t = t0; t < t1; t += BATCH) {
+            uint32_t e[BATCH];
+            int pj[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                e[u] = kNoRow;
+                pj[u] = 0;
+                const int tt = t + u;
+                if (tt < t1) {
+                    const int b = L.tto[tt];
+                    if (!((m >> b) & 1)) {
+                        const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
+                        e[u] = table[S[ib * SST] + L.bin[b * c.bs + ib + 1]];
+                        if (FERMI) pj[u] = (ib - ia) & 1;
+                    }
+                }
+            }
+            T v[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                v[u].x = 0;
+                v[u].y = 0;
+                if (e[u] != kNoRow) v[u] = xhat[e[u] >> c.shift];
+            }
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                if (e[u] == kNoRow) continue;
+                const int g = (e[u] >> 1) & c.gmask;
+                const T cf = cmul(L.tcoef[t + u], L.chi[g]);
+                const T pr = cmul(cf, v[u]);
+                if (((e[u] & 1) ^ pj[u]) != 0) {
+                    ax -= pr.x;
+                    ay -= pr.y;
+                } else {
+                    ax += pr.x;
+                    ay += pr.y;
+                }
+            }
+        }
+        ia += LPR;
+    }
+}
+
+}  // namespace sed

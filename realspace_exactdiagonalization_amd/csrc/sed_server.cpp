@@ -12,6 +12,7 @@
 #include <deque>
 #include <mutex>
 #include <thread>
+#include <memory>
 #include <unordered_map>
 
 #include "sed_internal.hpp"
@@ -47,6 +48,8 @@ struct Lane {
 struct sed_server {
     sed_context* ctx = nullptr;
     int64_t budget = 0;
+    sed::PackEngine* pack = nullptr;  // small sectors: packed persistent launches
+    std::unordered_map<int64_t, std::unique_ptr<sed::PackJob>> pack_jobs;
     std::mutex mu;
     std::condition_variable cv_lane, cv_done;
     std::deque<int64_t> queue;
@@ -115,6 +118,25 @@ void lane_main(sed_server* srv, Lane* lane) {
     }
 }
 
+// completion of a packed request (called from the packed engine's thread)
+void pack_done(void* user, PackJob* job) {
+    auto* srv = static_cast<sed_server*>(user);
+    std::lock_guard<std::mutex> lk(srv->mu);
+    Request& r = srv->reqs[job->id];
+    r.evals = job->evals;
+    r.stats = job->stats;
+    r.status = job->status;
+    r.error = job->error;
+    r.t_end = clk::now();
+    ++srv->n_done;
+    srv->cv_done.notify_all();
+}
+
+bool pack_disabled() {
+    const char* e = std::getenv("SED_PACKED");
+    return e && e[0] == '0';
+}
+
 }  // namespace
 }  // namespace sed
 
@@ -146,6 +168,15 @@ int sed_server_create(sed_context* ctx, int64_t hbm_budget_bytes, int max_slots,
         srv->lanes.push_back(lane);
     }
     for (Lane* lane : srv->lanes) lane->th = std::thread(lane_main, srv, lane);
+    if (!pack_disabled()) {
+        const int64_t arena = std::min<int64_t>(srv->budget / 4, (int64_t)16 << 30);
+        srv->pack = pack_create(ctx, arena, 512, pack_done, srv);
+        if (!srv->pack) {
+            const std::string err = sed_last_error();
+            sed_server_destroy(srv);
+            return fail(SED_ENOMEM, "packed engine: " + err);
+        }
+    }
     *out = srv;
     return SED_OK;
 }
@@ -157,7 +188,7 @@ int sed_server_submit(sed_server* srv, sed_sector* sec, int nev, double tol, uin
     if (precision != SED_PREC_FP64 && precision != SED_PREC_FP32 && precision != SED_PREC_FP32_REFINE)
         return fail(SED_EINVAL, "bad precision");
     if (!sec->has_terms) return fail(SED_ESTATE, "submit before sed_sector_set_terms");
-    std::lock_guard<std::mutex> lk(srv->mu);
+    std::unique_lock<std::mutex> lk(srv->mu);
     Request r;
     r.id = srv->next_id++;
     r.sec = sec;
@@ -166,9 +197,30 @@ int sed_server_submit(sed_server* srv, sed_sector* sec, int nev, double tol, uin
     r.seed = seed;
     r.maxiter = maxiter < 1 ? 300 : maxiter;
     r.precision = precision;
-    r.footprint = solve_footprint(sec, nev, precision);
     r.t_submit = clk::now();
     ++srv->n_submitted;
+    if (srv->pack && pack_eligible(sec, nev)) {
+        // small sector: joins the packed persistent launches at the next check point
+        r.status = 2;
+        r.t_start = r.t_submit;
+        r.footprint = pack_footprint(sec, nev, precision);
+        auto job = std::make_unique<PackJob>();
+        job->id = r.id;
+        job->sec = sec;
+        job->nev = nev;
+        job->tol = tol;
+        job->seed = seed;
+        job->maxiter = r.maxiter;
+        job->precision = precision;
+        PackJob* jp = job.get();
+        srv->pack_jobs[r.id] = std::move(job);
+        srv->reqs[r.id] = r;
+        *request_id = r.id;
+        lk.unlock();
+        pack_submit(srv->pack, jp);
+        return SED_OK;
+    }
+    r.footprint = solve_footprint(sec, nev, precision);
     if (r.footprint > srv->budget) {
         r.status = SED_ENOMEM;
         r.error = "request footprint " + std::to_string(r.footprint) + " exceeds the HBM budget";
@@ -232,6 +284,8 @@ int sed_server_destroy(sed_server* srv) {
         srv->stop = true;
         srv->cv_lane.notify_all();
     }
+    if (srv->pack) pack_destroy(srv->pack);   // finishes or cancels its requests
+    srv->pack = nullptr;
     for (Lane* lane : srv->lanes) {
         if (lane->th.joinable()) lane->th.join();
         (void)hipSetDevice(srv->ctx->device);

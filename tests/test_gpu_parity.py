@@ -270,3 +270,37 @@ def test_lanczos_graph_replay_matches_eager(ed, ctx, monkeypatch):
         res[mode] = out
     for a, b in zip(res["0"], res["1"]):
         assert np.array_equal(a, b)
+
+
+def test_packed_engine_matches_lane_path(ed, ctx, monkeypatch):
+    """Small sectors solved in the packed persistent launches give the eigenvalues of the
+    per-sector lane path (FP64 and FP32+refine), for bosons and fermions."""
+    cases = [(L.haldane_honeycomb(3, 4), (0.3, 0.0)), (L.checkerboard(3, 4, n_filled=6), (0.0, 0.21))]
+    bases, keep = [], []
+    for base, th in cases:
+        m = L.twisted(base, th)
+        G = L.model_group(m, th)
+        cat = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                              particle_statistics=m.statistics, ctx=ctx)
+        keep.append(cat)
+        for ir in L.translation_irreps(G, m.lattice)[:5]:
+            b = ed.build_symmetry_sector_basis(cat, ir)
+            if b.dim <= 500:
+                continue
+            b.set_terms(m.bilinear_terms, m.density_terms)
+            bases.append(b)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("SED_PACKED", mode)
+        srv = ed.SectorServer(ctx, max_slots=4)
+        ids = [(srv.submit(b, 3, precision=p), p) for b in bases for p in (ed.PREC_FP64, ed.PREC_FP32_REFINE)]
+        srv.run(0)
+        res = []
+        for rid, p in ids:
+            st, ev, stats, lat = srv.result(rid)
+            assert st == 0, (mode, rid)
+            res.append(ev)
+        srv.close()
+        out[mode] = res
+    for a, b in zip(out["0"], out["1"]):
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(a).max()
