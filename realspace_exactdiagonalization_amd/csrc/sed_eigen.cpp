@@ -152,29 +152,70 @@ void hermitian_eigh(int n, std::vector<cd>& A, std::vector<double>& evals, std::
     }
 }
 
+// Real symmetric n x n (row-major a, destroyed): Householder reduction to tridiagonal
+// form with the orthogonal factor accumulated, then implicit QL on it.  Eigenvalues
+// ascending; evecs row-major, column j = eigenvector j.
 void jacobi_eigh(int n, std::vector<double>& a, std::vector<double>& evals, std::vector<double>& evecs) {
-    // Real symmetric input (row-major) through the Hermitian routine.
-    std::vector<cd> A((size_t)n * n);
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) A[(size_t)j * n + i] = cd(a[(size_t)i * n + j], 0.0);
-    std::vector<cd> X;
-    hermitian_eigh(n, A, evals, &X);
-    evecs.assign((size_t)n * n, 0.0);
-    // X is column-major complex; for real symmetric input the vectors are real up to a
-    // global phase per column: rotate each column so its largest entry is real.
-    for (int j = 0; j < n; ++j) {
-        int imax = 0;
-        double vmax = -1.0;
-        for (int i = 0; i < n; ++i) {
-            const double m = std::abs(X[(size_t)j * n + i]);
-            if (m > vmax) {
-                vmax = m;
-                imax = i;
-            }
+    auto A = [&](int i, int j) -> double& { return a[(size_t)i * n + j]; };
+    std::vector<std::vector<double>> vs;
+    std::vector<double> taus, v(n), p(n), q(n);
+    for (int k = 0; k + 2 < n; ++k) {
+        double sigma2 = 0.0;
+        for (int i = k + 1; i < n; ++i) sigma2 += A(i, k) * A(i, k);
+        const double x0 = A(k + 1, k);
+        if (sigma2 - x0 * x0 <= 0.0) {
+            vs.emplace_back();
+            taus.push_back(0.0);
+            continue;
         }
-        const cd piv = X[(size_t)j * n + imax];
-        const cd rot = vmax > 0 ? std::conj(piv) / vmax : cd(1.0);
-        for (int i = 0; i < n; ++i) evecs[(size_t)i * n + j] = (X[(size_t)j * n + i] * rot).real();
+        const double sigma = std::sqrt(sigma2);
+        const double alpha = x0 >= 0 ? -sigma : sigma;
+        std::fill(v.begin(), v.end(), 0.0);
+        for (int i = k + 1; i < n; ++i) v[i] = A(i, k);
+        v[k + 1] -= alpha;
+        const double tau = 2.0 / (2.0 * sigma * (sigma + std::fabs(x0)));
+        for (int i = k + 1; i < n; ++i) {
+            double sm = 0.0;
+            for (int j = k + 1; j < n; ++j) sm += A(i, j) * v[j];
+            p[i] = tau * sm;
+        }
+        double vp = 0.0;
+        for (int i = k + 1; i < n; ++i) vp += v[i] * p[i];
+        const double K = 0.5 * tau * vp;
+        for (int i = k + 1; i < n; ++i) q[i] = p[i] - K * v[i];
+        for (int i = k + 1; i < n; ++i)
+            for (int j = k + 1; j < n; ++j) A(i, j) -= v[i] * q[j] + q[i] * v[j];
+        A(k + 1, k) = alpha;
+        A(k, k + 1) = alpha;
+        for (int i = k + 2; i < n; ++i) A(i, k) = A(k, i) = 0.0;
+        vs.push_back(v);
+        taus.push_back(tau);
+    }
+    std::vector<double> d(n), e(std::max(n, 1), 0.0);
+    for (int i = 0; i < n; ++i) d[i] = A(i, i);
+    for (int i = 0; i + 1 < n; ++i) e[i] = A(i + 1, i);
+    // Q = H_0 H_1 ... applied to the identity (row-major)
+    std::vector<double> z((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;
+    for (int k = (int)vs.size() - 1; k >= 0; --k) {
+        if (taus[k] == 0.0) continue;
+        const std::vector<double>& vk = vs[k];
+        for (int j = 0; j < n; ++j) {
+            double sm = 0.0;
+            for (int i = k + 1; i < n; ++i) sm += vk[i] * z[(size_t)i * n + j];
+            sm *= taus[k];
+            for (int i = k + 1; i < n; ++i) z[(size_t)i * n + j] -= vk[i] * sm;
+        }
+    }
+    tridiag_ql(n, d, e, &z);
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return d[x] < d[y]; });
+    evals.resize(n);
+    evecs.assign((size_t)n * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        evals[j] = d[order[j]];
+        for (int i = 0; i < n; ++i) evecs[(size_t)i * n + j] = z[(size_t)i * n + order[j]];
     }
 }
 

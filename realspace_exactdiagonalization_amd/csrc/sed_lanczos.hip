@@ -980,6 +980,15 @@ __global__ void __launch_bounds__(kPackBlock) k_pack_lanczos(const PackItem* __r
     if (threadIdx.x == 0) state[0] = j;
 }
 
+// step/flag/scale of a packed sector set by a kernel (no host temporaries in flight)
+__global__ void k_set_state(int* state, int step, int reset_flag, double* scale, double sval) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        state[0] = step;
+        if (reset_flag) state[1] = 0;
+        *scale = sval;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // device arena: one allocation, first-fit sub-allocation (256-byte aligned)
 // ---------------------------------------------------------------------------
@@ -1125,11 +1134,12 @@ class PackEngine {
         cb_(user_, job);
     }
 
-    // start vector of the current phase: random (FP64-only or FP32 phase) or v0 (refinement)
+    // start vector of the current phase: random (FP64-only or FP32 phase) or v0 (refinement);
+    // only launches (no synchronisation)
     template <typename T>
     int start_phase(PackReq& q, T* V, T* xh, bool from_v0) {
         double* sl = slot_ptr(q.slot);
-        int* step = reinterpret_cast<int*>(sl + 2 * q.m + 1);
+        int* state = reinterpret_cast<int*>(sl + 2 * q.m + 1);
         double* scale = sl + 2 * q.m;
         double* normpart = q.S;   // scratch: kUpdGrid doubles fit in the Ritz area
         if (from_v0)
@@ -1137,15 +1147,13 @@ class PackEngine {
                                reinterpret_cast<const double2*>(q.v0), V, q.n);
         else
             hipLaunchKernelGGL(k_random<T>, dim3(grid_for(q.n)), dim3(256), 0, st_, V, q.n, q.job->seed);
-        const int zero[2] = {0, 0};
-        SED_HIP(hipMemcpyAsync(step, zero, sizeof(zero), hipMemcpyHostToDevice, st_));
+        hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, st_, state, 0, 1, scale, 1.0);
         const unsigned g = (unsigned)std::min<int64_t>(kUpdGrid, std::max<int64_t>(1, (q.n + 255) / 256));
         hipLaunchKernelGGL(k_normsq<T>, dim3(g), dim3(256), 0, st_, V, q.n, normpart);
-        hipLaunchKernelGGL(k_start_finalize, dim3(1), dim3(kRedBlock), 0, st_, step, 0, normpart, (int)g, scale);
-        hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(q.n)), dim3(256), 0, st_, V, q.n, step, scale,
+        hipLaunchKernelGGL(k_start_finalize, dim3(1), dim3(kRedBlock), 0, st_, state, 0, normpart, (int)g, scale);
+        hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(q.n)), dim3(256), 0, st_, V, q.n, state, scale,
                            q.job->sec->d_stab.p, xh);
         SED_HIP(hipGetLastError());
-        SED_HIP(hipStreamSynchronize(st_));   // `zero` is a stack temporary
         q.T.assign((size_t)q.m * q.m, 0.0);
         q.k = 0;
         q.j = 0;
@@ -1185,23 +1193,29 @@ class PackEngine {
         return rc == SED_OK ? 0 : rc;
     }
 
+    // Ritz combination in place; Sk must stay alive until the stream is synchronised
     template <typename T>
-    int ritz(PackReq& q, T* V, int jm, const std::vector<double>& Sm, int kk) {
-        std::vector<double> Sk((size_t)jm * kk);
-        for (int a = 0; a < jm; ++a)
-            for (int i = 0; i < kk; ++i) Sk[(size_t)a * kk + i] = Sm[(size_t)a * jm + i];
+    int ritz(PackReq& q, T* V, int jm, const std::vector<double>& Sk, int kk) {
         SED_HIP(hipMemcpyAsync(q.S, Sk.data(), Sk.size() * 8, hipMemcpyHostToDevice, st_));
         const size_t lds = (((size_t)jm * kk * 8 + 15) & ~(size_t)15) + (size_t)jm * kRitzRows * sizeof(T);
         hipLaunchKernelGGL(k_ritz<T>, dim3((unsigned)((q.n + kRitzRows - 1) / kRitzRows)), dim3(kRitzRows), lds, st_, V,
                            q.n, jm, q.S, kk, q.n);
         SED_HIP(hipGetLastError());
-        SED_HIP(hipStreamSynchronize(st_));   // Sk is a host temporary
         return SED_OK;
     }
 
-    // host side of one check point for one request; returns 1 when the request left the engine
-    template <typename T>
-    int check(PackReq& q, T* V, T* xh) {
+    // What a check point decided for one request (pure host computation).
+    struct Decision {
+        int action = 0;          // 0 continue, 1 done, 2 refine (FP32 -> FP64), 3 restart, -1 fail
+        int rc = SED_OK;
+        std::string err;
+        int jm = 0, kk = 0;
+        std::vector<double> theta, Sk;   // Ritz values; kept columns of S (jm x kk)
+        double beta = 0.0, resmax = 0.0;
+        int conv = 0;
+    };
+
+    void decide(PackReq& q, Decision& d) {
         PackJob* job = q.job;
         const int m = q.m;
         const double* hs = h_slots_ + (size_t)q.slot * slot_doubles_;
@@ -1210,8 +1224,7 @@ class PackEngine {
         int st2[2];
         std::memcpy(st2, hs + 2 * m + 1, sizeof(st2));
         const int jdev = st2[0], flag = st2[1];
-        const int done_steps = jdev - q.j;
-        q.iters[q.phase] += done_steps;
+        q.iters[q.phase] += jdev - q.j;
         q.j = jdev;
         int jm = jdev;
         bool breakdown = false;
@@ -1227,75 +1240,103 @@ class PackEngine {
             }
         }
         const double beta = breakdown ? 0.0 : hb[jm - 1];
-        std::vector<double> a((size_t)jm * jm), theta, Sm;
+        std::vector<double> a((size_t)jm * jm), Sm;
         for (int r = 0; r < jm; ++r)
             for (int c = 0; c < jm; ++c) a[(size_t)r * jm + c] = q.T[(size_t)r * m + c];
-        jacobi_eigh(jm, a, theta, Sm);
+        jacobi_eigh(jm, a, d.theta, Sm);
         const int nev = std::min(job->nev, jm);
         const double tol = q.phase == 0 ? (job->precision == SED_PREC_FP32 ? std::max(job->tol, 1e-5) : 1e-4)
                                         : job->tol;
-        int conv = 0;
-        double resmax = 0.0;
         for (int i = 0; i < nev; ++i) {
             const double res = beta * std::fabs(Sm[(size_t)(jm - 1) * jm + i]);
-            if (res <= tol && conv == i) ++conv;
-            resmax = std::max(resmax, res);
+            if (res <= tol && d.conv == i) ++d.conv;
+            d.resmax = std::max(d.resmax, res);
         }
-        const int total_iters = q.iters[q.phase];
-        if (conv >= job->nev || breakdown) {
-            if (conv < job->nev) {
-                finish(q, SED_ENOCONV, "Lanczos (packed) did not converge the requested eigenpairs");
-                return 1;
+        d.jm = jm;
+        d.beta = beta;
+        auto keep_cols = [&](int kk) {
+            d.kk = kk;
+            d.Sk.resize((size_t)jm * kk);
+            for (int r = 0; r < jm; ++r)
+                for (int i = 0; i < kk; ++i) d.Sk[(size_t)r * kk + i] = Sm[(size_t)r * jm + i];
+        };
+        if (d.conv >= job->nev || breakdown) {
+            if (d.conv < job->nev) {
+                d.action = -1;
+                d.rc = SED_ENOCONV;
+                d.err = "Lanczos (packed) did not converge the requested eigenpairs";
+            } else if (q.phase == 0 && job->precision == SED_PREC_FP32_REFINE) {
+                d.action = 2;
+                keep_cols(nev);
+            } else {
+                d.action = 1;
             }
-            if (q.phase == 0 && job->precision == SED_PREC_FP32_REFINE) {
-                // FP64 refinement restarted from the sum of the FP32 Ritz vectors
-                SED_TRY(ritz<T>(q, V, jm, Sm, nev));
-                hipLaunchKernelGGL(k_sum_vectors<T>, dim3(grid_for(q.n)), dim3(256), 0, st_, V, q.n, nev,
+            return;
+        }
+        if (q.iters[q.phase] >= job->maxiter) {
+            d.action = -1;
+            d.rc = SED_ENOCONV;
+            d.err = "Lanczos (packed) reached maxiter";
+            return;
+        }
+        if (jm == m) {
+            d.action = 3;
+            keep_cols(std::min(m - 2, job->nev + std::max(4, (m - job->nev) / 3)));
+            // the restart's projected matrix: Ritz values and their couplings to v_m
+            std::fill(q.T.begin(), q.T.end(), 0.0);
+            for (int i = 0; i < d.kk; ++i) {
+                q.T[(size_t)i * m + i] = d.theta[i];
+                const double sc = beta * Sm[(size_t)(jm - 1) * jm + i];
+                q.T[(size_t)d.kk * m + i] = sc;
+                q.T[(size_t)i * m + d.kk] = sc;
+            }
+        }
+    }
+
+    // GPU side of a decision (launches only); returns 1 when the request left the engine
+    template <typename T>
+    int apply(PackReq& q, T* V, T* xh, Decision& d) {
+        PackJob* job = q.job;
+        switch (d.action) {
+            case -1:
+                finish(q, d.rc, d.err);
+                return 1;
+            case 1:
+                job->evals.assign(d.theta.begin(), d.theta.begin() + std::min<int>(job->nev, (int)d.theta.size()));
+                job->stats.converged = d.conv;
+                job->stats.residual_max = d.resmax;
+                finish(q, SED_OK, "");
+                return 1;
+            case 2: {
+                SED_TRY(ritz<T>(q, V, d.jm, d.Sk, d.kk));
+                hipLaunchKernelGGL(k_sum_vectors<T>, dim3(grid_for(q.n)), dim3(256), 0, st_, V, q.n, d.kk,
                                    reinterpret_cast<double2*>(q.v0), q.n);
                 SED_HIP(hipGetLastError());
                 q.phase = 1;
-                return start_phase<double2>(q, reinterpret_cast<double2*>(q.V64), reinterpret_cast<double2*>(q.xh64),
-                                            true) == SED_OK ? 0 : -1;
+                SED_TRY(start_phase<double2>(q, reinterpret_cast<double2*>(q.V64), reinterpret_cast<double2*>(q.xh64),
+                                             true));
+                return 0;
             }
-            job->evals.assign(theta.begin(), theta.begin() + nev);
-            job->stats.converged = conv;
-            job->stats.residual_max = resmax;
-            finish(q, SED_OK, "");
-            return 1;
-        }
-        if (total_iters >= job->maxiter) {
-            finish(q, SED_ENOCONV, "Lanczos (packed) reached maxiter");
-            return 1;
-        }
-        if (jm == m) {
-            // thick restart: kk Ritz vectors + the residual direction v_m
-            const int kk = std::min(m - 2, job->nev + std::max(4, (m - job->nev) / 3));
-            SED_TRY(ritz<T>(q, V, jm, Sm, kk));
-            SED_HIP(hipMemcpyAsync(V + (size_t)kk * q.n, V + (size_t)jm * q.n, (size_t)q.n * sizeof(T),
-                                   hipMemcpyDeviceToDevice, st_));
-            double* sl = slot_ptr(q.slot);
-            int* step = reinterpret_cast<int*>(sl + 2 * m + 1);
-            double* scale = sl + 2 * m;
-            const int jk[1] = {kk};
-            const double one = 1.0;
-            SED_HIP(hipMemcpyAsync(step, jk, sizeof(int), hipMemcpyHostToDevice, st_));
-            SED_HIP(hipMemcpyAsync(scale, &one, sizeof(double), hipMemcpyHostToDevice, st_));
-            hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(q.n)), dim3(256), 0, st_, V, q.n, step, scale,
-                               job->sec->d_stab.p, xh);
-            SED_HIP(hipGetLastError());
-            SED_HIP(hipStreamSynchronize(st_));
-            std::fill(q.T.begin(), q.T.end(), 0.0);
-            for (int i = 0; i < kk; ++i) {
-                q.T[(size_t)i * m + i] = theta[i];
-                const double sc = beta * Sm[(size_t)(jm - 1) * jm + i];
-                q.T[(size_t)kk * m + i] = sc;
-                q.T[(size_t)i * m + kk] = sc;
+            case 3: {
+                const int m = q.m, kk = d.kk;
+                SED_TRY(ritz<T>(q, V, d.jm, d.Sk, kk));
+                SED_HIP(hipMemcpyAsync(V + (size_t)kk * q.n, V + (size_t)d.jm * q.n, (size_t)q.n * sizeof(T),
+                                       hipMemcpyDeviceToDevice, st_));
+                double* sl = slot_ptr(q.slot);
+                int* state = reinterpret_cast<int*>(sl + 2 * m + 1);
+                double* scale = sl + 2 * m;
+                hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, st_, state, kk, 0, scale, 1.0);
+                hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(q.n)), dim3(256), 0, st_, V, q.n, state, scale,
+                                   job->sec->d_stab.p, xh);
+                SED_HIP(hipGetLastError());
+                q.k = kk;
+                q.j = kk;
+                ++q.restarts;
+                return 0;
             }
-            q.k = kk;
-            q.j = kk;
-            ++q.restarts;
+            default:
+                return 0;
         }
-        return 0;
     }
 
     template <typename M, typename R, bool F>
@@ -1405,23 +1446,53 @@ class PackEngine {
                 break;
             }
             if (act.empty()) continue;
+            const auto tc0 = std::chrono::steady_clock::now();
             const int rc = cycle(act);
+            const auto tc1 = std::chrono::steady_clock::now();
+            if (rc != SED_OK) {
+                const std::string err = sed_last_error();
+                for (PackReq& q : act) finish(q, rc, err);
+                act.clear();
+                continue;
+            }
+            // host decisions in parallel (pure computation), then the GPU actions in order
+            std::vector<Decision> dec(act.size());
+            {
+                const int nthr = (int)std::min<size_t>(act.size(), (size_t)std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+                std::vector<std::thread> pool;
+                for (int t = 0; t < nthr; ++t)
+                    pool.emplace_back([&, t] {
+                        for (size_t i = t; i < act.size(); i += nthr) decide(act[i], dec[i]);
+                    });
+                for (auto& th : pool) th.join();
+            }
+            const auto tc2 = std::chrono::steady_clock::now();
             std::vector<PackReq> keep;
-            for (PackReq& q : act) {
-                if (rc != SED_OK) {
-                    finish(q, rc, sed_last_error());
-                    continue;
-                }
+            std::vector<int> done_idx;
+            for (size_t i = 0; i < act.size(); ++i) {
+                PackReq& q = act[i];
                 int r;
-                if (q.phase == 0) r = check<float2>(q, reinterpret_cast<float2*>(q.V32), reinterpret_cast<float2*>(q.xh32));
-                else r = check<double2>(q, reinterpret_cast<double2*>(q.V64), reinterpret_cast<double2*>(q.xh64));
+                if (q.phase == 0) r = apply<float2>(q, reinterpret_cast<float2*>(q.V32), reinterpret_cast<float2*>(q.xh32), dec[i]);
+                else r = apply<double2>(q, reinterpret_cast<double2*>(q.V64), reinterpret_cast<double2*>(q.xh64), dec[i]);
                 if (r < 0) finish(q, r, sed_last_error());
                 else if (r == 0) keep.push_back(q);
             }
+            // the Ritz coefficients in `dec` must outlive the copies queued above
+            if (hipStreamSynchronize(st_) != hipSuccess) {
+                for (PackReq& q : keep) finish(q, SED_EDEVICE, "packed engine stream failed");
+                keep.clear();
+            }
             act.swap(keep);
+            if (trace_) {
+                const auto tc3 = std::chrono::steady_clock::now();
+                auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+                fprintf(stderr, "[pack] active=%zu gpu=%.2fms decide=%.2fms apply=%.2fms\n", dec.size(), ms(tc0, tc1),
+                        ms(tc1, tc2), ms(tc2, tc3));
+            }
         }
     }
 
+    bool trace_ = std::getenv("SED_PACK_TRACE") != nullptr;
     sed_context* ctx_;
     int max_active_;
     void (*cb_)(void*, PackJob*);
