@@ -35,11 +35,7 @@ static int upload(DevBuf<T>& b, const T* h, size_t n, hipStream_t s) {
     return SED_OK;
 }
 
-static int max_lds(const sed_context* ctx) {
-    hipDeviceProp_t p;
-    if (hipGetDeviceProperties(&p, ctx->device) != hipSuccess) return 65536;
-    return (int)p.sharedMemPerBlock;
-}
+static int max_lds(const sed_context* ctx) { return ctx->max_lds; }
 
 }  // namespace sed
 
@@ -63,6 +59,7 @@ int sed_create(int device, sed_context** out) {
     auto* ctx = new sed_context();
     ctx->device = device;
     ctx->n_cu = p.multiProcessorCount;
+    ctx->max_lds = (int)p.sharedMemPerBlock;
     ctx->arch = p.gcnArchName;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {

@@ -63,6 +63,7 @@ struct sed_context {
     int device = 0;
     hipStream_t stream = nullptr;
     int n_cu = 0;
+    int max_lds = 65536;              // LDS bytes per workgroup
     std::string arch;
     sed::Workspace* ws = nullptr;     // workspace of the synchronous API (on `stream`)
 };
