@@ -137,6 +137,155 @@ def flux_scan(E, L, ctx, n_twists=16, nev=3, lanes=8):
     return out
 
 
+# Serving-stream request families: XXZ chains at half filling, 2 x L Hubbard ladders at
+# quarter filling (U = 4), checkerboard lattices at 1/3 filling of the lower band.
+STREAM_SIZES = {"xxz": [20, 22, 24, 26, 28, 30, 32], "ladder": [6, 7, 8, 9],
+                "checker": [(3, 4), (3, 5), (3, 6), (4, 4), (4, 5), (4, 6)]}
+
+
+def stream_model(L, family, size):
+    if family == "xxz":
+        return L.heisenberg_chain(size)
+    if family == "ladder":
+        return L.hubbard_square(size, 2, U=4.0, t=1.0, n_filled=size)
+    return L.checkerboard(size[0], size[1])
+
+
+def stream_requests(n, seed):
+    """Seeded request list: family and size uniform, momentum draw u in [0, 1) (irrep index
+    floor(u * |G|)), twist uniform in [0, 1)^2 (the chain uses only its x component)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    fams = ["xxz", "ladder", "checker"]
+    out = []
+    for _ in range(n):
+        f = fams[int(rng.integers(3))]
+        sizes = STREAM_SIZES[f]
+        size = sizes[int(rng.integers(len(sizes)))]
+        u = float(rng.random())
+        th = (float(rng.random()), float(rng.random()))
+        if f == "xxz":
+            th = (th[0], 0.0)
+        out.append({"family": f, "size": size, "u": u, "theta": th})
+    return out
+
+
+def poisson_arrivals(n, rate, seed):
+    """Arrival times (s) of a Poisson stream of n requests at `rate` per second, first at 0."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    return t - t[0]
+
+
+def serving_stream(E, L, ctx, n_req=32, seed=2024, nev=3, lanes=8, load=1.2):
+    """Config[4]-style serving stream.  The sectors of all requests are built first (tables
+    resident in HBM); then (1) the plain FP64 path solves them one after another, (2) the
+    continuous-batching server (FP32 + FP64 refinement) takes all of them at once, which
+    measures its saturated throughput, and (3) the server is fed a seeded Poisson stream at
+    `load` times that throughput.  The plain path's latencies for the same arrivals follow
+    from its solve times (one FIFO server).  Reports solves/s and p50/p99 latency."""
+    import threading
+    import numpy as np
+    reqs = stream_requests(n_req, seed)
+    t0 = time.time()
+    keep, sectors, info = [], [], []
+    for r in reqs:
+        m = L.twisted(stream_model(L, r["family"], r["size"]), r["theta"])
+        G = L.model_group(m, r["theta"])
+        cat = E.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                             particle_statistics=m.statistics, ctx=ctx)
+        irs = L.translation_irreps(G, m.lattice)
+        b = E.build_symmetry_sector_basis(cat, irs[min(len(irs) - 1, int(r["u"] * len(irs)))])
+        b.set_terms(m.bilinear_terms, m.density_terms)
+        keep.append(cat)
+        if b.dim == 0:
+            b.close()
+            continue
+        sectors.append(b)
+        info.append((r["family"], r["size"], b.dim))
+    t_build = time.time() - t0
+    n = len(sectors)
+    dims = np.array([d for _, _, d in info])
+    # (1) plain FP64, one solve after another
+    ev_plain, t_solve = [], []
+    for b in sectors:
+        H = E.MatrixFreeHamiltonian.__new__(E.MatrixFreeHamiltonian)
+        H.basis, H.n, H.precision = b, b.dim, E.PREC_FP64
+        t1 = time.time()
+        ev_plain.append(E.diagonalize_block_matrixfree(H, b.dim, nev=min(nev, b.dim), return_vectors=False)[0])
+        t_solve.append(time.time() - t1)
+    t_solve = np.array(t_solve)
+    plain_rate = n / float(t_solve.sum())
+
+    def collect(srv, ids):
+        lat, err = [], 0.0
+        for rid, ref in zip(ids, ev_plain):
+            st, ev, stats, l = srv.result(rid)
+            if st != 0:
+                raise RuntimeError(f"stream request {rid} failed with status {st}")
+            lat.append(l)
+            ev = ev[:len(ref)]
+            err = max(err, float(np.max(np.abs(ev - ref) / np.maximum(np.abs(ref), 1e-300))))
+        return np.array(lat), err
+
+    # (2) saturated throughput of the server
+    srv = E.SectorServer(ctx, max_slots=lanes)
+    t1 = time.time()
+    ids = [srv.submit(b, min(nev, b.dim), precision=E.PREC_FP32_REFINE) for b in sectors]
+    srv.run(0)
+    sat_wall = time.time() - t1
+    _, err_sat = collect(srv, ids)
+    srv.close()
+    sat_rate = n / sat_wall
+    # (3) Poisson stream at load x the saturated rate
+    arr = poisson_arrivals(n, load * sat_rate, seed + 1)
+    srv = E.SectorServer(ctx, max_slots=lanes)
+    ids = [None] * n
+
+    def feed():
+        t_ref = time.time()
+        for i, b in enumerate(sectors):
+            dt = t_ref + arr[i] - time.time()
+            if dt > 0:
+                time.sleep(dt)
+            ids[i] = srv.submit(b, min(nev, b.dim), precision=E.PREC_FP32_REFINE)
+
+    t1 = time.time()
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    srv.run(0)
+    wall = time.time() - t1
+    lat, err = collect(srv, ids)
+    srv.close()
+    # plain FP64 on the same arrivals: single FIFO server
+    done = np.zeros(n)
+    tcur = 0.0
+    for i in range(n):
+        tcur = max(tcur, arr[i]) + t_solve[i]
+        done[i] = tcur
+    plain_lat = done - arr
+    for b in sectors:
+        b.close()
+    q = lambda a, p: round(1e3 * float(np.percentile(a, p)), 2)
+    return {
+        "requests": n, "families": {f: int(sum(1 for x in info if x[0] == f)) for f in STREAM_SIZES},
+        "dims": [int(dims.min()), int(np.median(dims)), int(dims.max())], "nev": nev, "seed": seed,
+        "build_s": round(t_build, 2),
+        "plain_fp64": {"solves_per_s": round(plain_rate, 3), "solve_s_total": round(float(t_solve.sum()), 3),
+                       "stream_solves_per_s": round(n / float(done[-1]), 3),
+                       "p50_ms": q(plain_lat, 50), "p99_ms": q(plain_lat, 99)},
+        "server_fp32_refine": {"saturated_solves_per_s": round(sat_rate, 3),
+                               "arrival_rate_per_s": round(load * sat_rate, 3),
+                               "stream_solves_per_s": round(n / wall, 3),
+                               "p50_ms": q(lat, 50), "p99_ms": q(lat, 99),
+                               "max_rel_diff_vs_plain_fp64": max(err, err_sat), "lanes": lanes},
+        "speedup_saturated_vs_plain_fp64": round(sat_rate / plain_rate, 3),
+        "per_request_plain_s": [[f, str(sz), int(d), round(float(t), 3)] for (f, sz, d), t in zip(info, t_solve)],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,7 +295,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flux", action="store_true", help="skip the batched flux-scan solves")
-    ap.add_argument("--lanes", type=int, default=8, help="server lanes for the flux scan")
+    ap.add_argument("--lanes", type=int, default=8, help="server lanes for the flux scan and the stream")
+    ap.add_argument("--no-stream", action="store_true", help="skip the serving-stream benchmark")
+    ap.add_argument("--stream-requests", type=int, default=32)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -208,6 +359,14 @@ def main():
               f"batched fp64 {flux['batched_fp64']['solves_per_s']}/s, batched fp32+refine "
               f"{flux['batched_fp32_refine']['solves_per_s']}/s (p99 {flux['batched_fp32_refine']['p99_ms']} ms)",
               flush=True)
+    stream = None
+    if not args.no_stream:
+        stream = serving_stream(E, L, ctx, n_req=args.stream_requests, lanes=args.lanes)
+        print(f"[bench] serving stream {stream['requests']} requests dims {stream['dims']}: plain fp64 "
+              f"{stream['plain_fp64']['solves_per_s']}/s, server fp32+refine saturated "
+              f"{stream['server_fp32_refine']['saturated_solves_per_s']}/s, Poisson stream p50/p99 "
+              f"{stream['server_fp32_refine']['p50_ms']}/{stream['server_fp32_refine']['p99_ms']} ms "
+              f"(plain {stream['plain_fp64']['p50_ms']}/{stream['plain_fp64']['p99_ms']} ms)", flush=True)
     main_r = res["c32"]
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     cpu = None if args.no_cpu else cpu_baseline(model, G, ir, cat, basis, args.cpu_seconds, threads,
@@ -235,6 +394,7 @@ def main():
                  "frac": round(res["c64"]["gbs"] / PEAK_HBM_GBS, 4), "launch_shape": res["c64"]["shape"]},
         "launch_shape": res["c32"]["shape"],
         "flux_scan_haldane3x4": flux,
+        "serving_stream": stream,
         "setup_s": {"catalog": round(t_cat, 3), "sector": round(t_sec, 3), "total": round(time.time() - t_setup, 3)},
     }
     print(json.dumps(out), flush=True)

@@ -67,6 +67,9 @@ int sed_version(void);
 /* One context per GPU (device ordinal), owning a HIP stream.  No reference
  * counterpart: the reference runs on CPU threads. */
 int sed_create(int device, sed_context** out);
+/* Destroy calls may come in any order: a context still used by live catalogs or
+ * servers, and a catalog still used by live sectors, is released together with the
+ * last object that uses it. */
 int sed_destroy(sed_context* ctx);
 
 /* build_symmetry_orbit_catalog(; n_filled, symmetry_group, particle_statistics)
@@ -131,7 +134,8 @@ int sed_sector_shape(const sed_sector* sec, int precision, int* shape, int* lane
  * may be NULL).  Sectors of dim <= 500 are solved densely, as the reference does;
  * larger ones by thick-restart Lanczos with full reorthogonalisation from a seeded
  * Gaussian start vector, residual tolerance tol (reference: 1e-10), at most maxiter
- * matvecs per precision phase.  Returns SED_ENOCONV if fewer than nev converge. */
+ * thick restarts per precision phase (the reference's KrylovKit maxiter = 300 likewise
+ * counts restarts of its Krylov basis).  Returns SED_ENOCONV if fewer than nev converge. */
 int sed_solve(sed_sector* sec, int nev, double tol, uint64_t seed, int maxiter, int precision,
               double* evals, double* evecs, sed_solve_stats* stats);
 /* Site densities <n_i> of eigenvector `which` (0-based) of the last sed_solve on this

@@ -1,5 +1,6 @@
 // C ABI of libsed: handles, validation, host-side preparation of the device tables.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -36,6 +37,38 @@ static int upload(DevBuf<T>& b, const T* h, size_t n, hipStream_t s) {
 }
 
 static int max_lds(const sed_context* ctx) { return ctx->max_lds; }
+
+static std::atomic<int64_t> g_term_version{0};
+
+static void free_context(sed_context* ctx) {
+    (void)hipSetDevice(ctx->device);
+    if (ctx->ws) {
+        ctx->ws->release();
+        delete ctx->ws;
+    }
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+void context_child_added(sed_context* ctx) { ++ctx->n_children; }
+void context_child_removed(sed_context* ctx) {
+    if (--ctx->n_children == 0 && ctx->released) free_context(ctx);
+}
+
+static void free_catalog(sed_catalog* cat) {
+    sed_context* ctx = cat->ctx;
+    (void)hipSetDevice(ctx->device);
+    cat->d_binom.release();
+    cat->d_perm.release();
+    cat->d_inv.release();
+    cat->d_rep_rank.release();
+    cat->d_sym_par.release();
+    cat->d_orbit_rep.release();
+    cat->d_orbit_stab.release();
+    cat->d_orbit_stab_par.release();
+    delete cat;
+    context_child_removed(ctx);
+}
 
 }  // namespace sed
 
@@ -74,13 +107,11 @@ int sed_create(int device, sed_context** out) {
 
 int sed_destroy(sed_context* ctx) {
     if (!ctx) return SED_OK;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->ws) {
-        ctx->ws->release();
-        delete ctx->ws;
+    if (ctx->n_children > 0) {
+        ctx->released = true;   // freed with its last catalog / server
+        return SED_OK;
     }
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    free_context(ctx);
     return SED_OK;
 }
 
@@ -197,6 +228,7 @@ int sed_catalog_build(sed_context* ctx, int n_site, int n_filled, int statistics
     SED_TRY(upload(cat->d_inv, inv8.data(), inv8.size(), s));
     SED_TRY(launch_catalog(cat));
     *out = guard.release();
+    context_child_added(ctx);
     return SED_OK;
 }
 
@@ -291,16 +323,11 @@ int sed_catalog_stabilizers(const sed_catalog* cat, int32_t* stab_gidx, double* 
 
 int sed_catalog_destroy(sed_catalog* cat) {
     if (!cat) return SED_OK;
-    (void)hipSetDevice(cat->ctx->device);
-    cat->d_binom.release();
-    cat->d_perm.release();
-    cat->d_inv.release();
-    cat->d_rep_rank.release();
-    cat->d_sym_par.release();
-    cat->d_orbit_rep.release();
-    cat->d_orbit_stab.release();
-    cat->d_orbit_stab_par.release();
-    delete cat;
+    if (cat->n_sectors_alive > 0) {
+        cat->released = true;   // freed with its last sector
+        return SED_OK;
+    }
+    free_catalog(cat);
     return SED_OK;
 }
 
@@ -434,7 +461,9 @@ int sed_sector_set_terms(sed_sector* sec, int n_bil, const int32_t* i_from, cons
         return fail(SED_EUNSUPPORTED, "term lists need " + std::to_string(need) + " bytes of LDS per workgroup");
     sec->has_terms = true;
     sec->hops_gated = sec->hops_in_sector = -1;
-    ++sec->version;
+    // process-wide unique: a captured Lanczos step keyed on (sector address, version)
+    // can never match a later sector that reuses a destroyed sector's address
+    sec->version = ++g_term_version;
     return SED_OK;
 }
 
@@ -619,6 +648,9 @@ int sed_sector_destroy(sed_sector* sec) {
     if (!sec) return SED_OK;
     (void)hipSetDevice(sec->cat->ctx->device);
     --sec->cat->n_sectors_alive;
+    // the synchronous workspace's captured step references this sector's tables
+    Workspace* ws = sec->cat->ctx->ws;
+    if (ws && ws->graph_key.sec == sec) ws->drop_graph();
     for (auto* b : {&sec->d_rep}) b->release();
     sec->d_stab.release();
     sec->d_table.release();
@@ -633,7 +665,9 @@ int sed_sector_destroy(sed_sector* sec) {
     sec->d_dens_v32.release();
     sec->d_chi32.release();
     sec->d_gauge.release();
+    sed_catalog* cat = sec->cat;
     delete sec;
+    if (cat->n_sectors_alive == 0 && cat->released) free_catalog(cat);
     return SED_OK;
 }
 

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <complex>
 #include <cstdint>
 #include <string>
@@ -66,6 +68,8 @@ struct sed_context {
     int max_lds = 65536;              // LDS bytes per workgroup
     std::string arch;
     sed::Workspace* ws = nullptr;     // workspace of the synchronous API (on `stream`)
+    int64_t n_children = 0;           // live catalogs and servers
+    bool released = false;            // sed_destroy called while children were alive
 };
 
 struct sed_catalog {
@@ -89,6 +93,7 @@ struct sed_catalog {
     sed::DevBuf<uint64_t> d_orbit_stab;      // bit h set if h(rep) == rep
     sed::DevBuf<uint64_t> d_orbit_stab_par;  // bit h set if that stabilizer element has sign -1
     int64_t n_sectors_alive = 0;
+    bool released = false;            // sed_catalog_destroy called while sectors were alive
 };
 
 struct sed_sector {
@@ -157,12 +162,18 @@ struct Workspace {
     int ensure_host(size_t n_doubles);
     void drop_graph();
     void release();
+    size_t device_bytes() const;
 };
 template <typename T>
 int grow(DevBuf<T>& b, size_t n) {
     if (b.n >= n) return SED_OK;
     return b.alloc(n);
 }
+
+// Lifetime: a context outlives its catalogs and servers, a catalog its sectors.  Their
+// destroy calls defer the release until the last dependent object is gone.
+void context_child_added(sed_context* ctx);
+void context_child_removed(sed_context* ctx);
 
 // ---- kernel launchers (device code in the .hip files) ----
 int launch_catalog(sed_catalog* cat);
@@ -202,6 +213,17 @@ void pack_submit(PackEngine* e, PackJob* job);
 void pack_destroy(PackEngine* e);
 bool pack_eligible(const sed_sector* sec, int nev);
 int64_t pack_footprint(const sed_sector* sec, int nev, int precision);
+
+// True if two of the first nev FP32 Ritz values (ascending) coincide within the FP32
+// phase's accuracy.  A single start vector spans one direction of a degenerate
+// eigenspace, so the FP64 refinement then starts from the seeded random vector (as a
+// plain FP64 solve does) rather than from the sum of the FP32 Ritz vectors.
+inline bool fp32_cluster(const std::vector<double>& theta, int nev) {
+    const int k = std::min<int>(nev, (int)theta.size());
+    for (int i = 0; i + 1 < k; ++i)
+        if (std::abs(theta[i + 1] - theta[i]) <= 1e-5 * std::max(1.0, std::abs(theta[i]))) return true;
+    return false;
+}
 
 // ---- host eigensolvers (sed_eigen.cpp) ----
 // Real symmetric n x n (row-major, overwritten) -> ascending eigenvalues, eigenvectors in columns.

@@ -47,8 +47,10 @@ constexpr int kUpdGrid = 1024;  // fixed grid of the update / norm kernels
 // part[(j+1) * G + b] = sum |w[r]|^2.
 template <typename T>
 __global__ void __launch_bounds__(kRedBlock) k_dots(const T* __restrict__ V, int64_t n, const int* __restrict__ step,
-                                                    const T* __restrict__ w, double2* __restrict__ part) {
+                                                    const T* __restrict__ w, double2* __restrict__ part,
+                                                    const int* __restrict__ gate) {
     __shared__ double2 red[kDotGroup][kRedBlock / kWave];
+    if (gate && *gate == 0) return;   // second pass not needed this step
     const int nv = *step + 1;
     const int total = nv + 1;
     const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
@@ -93,8 +95,10 @@ __global__ void __launch_bounds__(kRedBlock) k_dots(const T* __restrict__ V, int
 
 // out[i] = sum_b part[i * G + b] for i <= j + 1 (fixed order); launched with m + 2 blocks.
 __global__ void __launch_bounds__(kRedBlock) k_reduce(const double2* __restrict__ part, int G,
-                                                      const int* __restrict__ step, double2* __restrict__ out) {
+                                                      const int* __restrict__ step, double2* __restrict__ out,
+                                                      const int* __restrict__ gate) {
     __shared__ double2 red[kRedBlock / kWave];
+    if (gate && *gate == 0) return;
     const int i = blockIdx.x;
     if (i > *step + 1) return;
     double2 acc = make_double2(0.0, 0.0);
@@ -116,17 +120,19 @@ __global__ void __launch_bounds__(kRedBlock) k_reduce(const double2* __restrict_
 }
 
 // out[r] = w[r] - sum_{i<=j} V_i[r] h[i]; out = w (to_next = 0) or V_{j+1} (to_next = 1).
-// With to_next, each block also writes the partial |out|^2 of its rows (fixed grid).
+// Each block also writes the partial |out|^2 of its rows (fixed grid).  A zero gate
+// (second pass skipped) makes the update a copy.
 template <typename T>
 __global__ void __launch_bounds__(256) k_update(T* V, int64_t n, const int* __restrict__ step,
                                                 const double2* __restrict__ h, T* w, int to_next,
-                                                double* __restrict__ normpart) {
+                                                double* __restrict__ normpart, const int* __restrict__ gate) {
     __shared__ double2 lh[kMaxBasis + 2];
     __shared__ double red[256 / kWave];
-    const int nv = *step + 1;
+    const int nv = (gate && *gate == 0) ? 0 : *step + 1;
+    const int nout = *step + 1;
     for (int i = threadIdx.x; i < nv; i += blockDim.x) lh[i] = h[i];
     __syncthreads();
-    T* out = to_next ? V + (int64_t)nv * n : w;
+    T* out = to_next ? V + (int64_t)nout * n : w;
     double nrm = 0.0;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
         double2 a = to_d2(w[r]);
@@ -141,7 +147,6 @@ __global__ void __launch_bounds__(256) k_update(T* V, int64_t n, const int* __re
         const double2 od = to_d2(o);
         nrm += od.x * od.x + od.y * od.y;
     }
-    if (!to_next) return;
     for (int off = 32; off > 0; off >>= 1) nrm += __shfl_down(nrm, off, kWave);
     if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nrm;
     __syncthreads();
@@ -150,6 +155,27 @@ __global__ void __launch_bounds__(256) k_update(T* V, int64_t n, const int* __re
         for (int k = 0; k < 256 / kWave; ++k) sm += red[k];
         normpart[blockIdx.x] = sm;
     }
+}
+
+// Single block, after the first pass: |w'|^2 from the update's partials against |w|^2
+// (h1[j+1]); gate = 1 when the second pass must run, else h2[j] = 0 for k_finalize.
+__global__ void __launch_bounds__(kRedBlock) k_reorth_check(const int* __restrict__ step,
+                                                            const double* __restrict__ normpart, int G,
+                                                            const double2* __restrict__ h1, double2* __restrict__ h2,
+                                                            int* __restrict__ gate) {
+    __shared__ double red[kRedBlock / kWave];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < G; b += kRedBlock) acc += normpart[b];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double nrm2 = 0.0;
+    for (int k = 0; k < kRedBlock / kWave; ++k) nrm2 += red[k];
+    const int j = *step;
+    const int need = nrm2 > kReorthEta2 * h1[j + 1].x ? 0 : 1;
+    *gate = need;
+    if (!need) h2[j] = make_double2(0.0, 0.0);
 }
 
 // Single block: |v_{j+1}|^2 from the update's partials, alpha_j, beta_j, 1/beta_j, and
@@ -322,6 +348,11 @@ void Workspace::release() {
     hcoef_n = 0;
 }
 
+size_t Workspace::device_bytes() const {
+    return V64.bytes() + w64.bytes() + xh64.bytes() + V32.bytes() + w32.bytes() + xh32.bytes() + part.bytes() +
+           coef.bytes() + S.bytes() + v0.bytes() + state.bytes();
+}
+
 template <typename T>
 struct VecTraits;
 template <>
@@ -393,6 +424,7 @@ class ThickLanczos {
         scale_ = beta_ + m_;
         step_ = reinterpret_cast<int*>(scale_ + 1);
         flag_ = step_ + 1;
+        gate_ = step_ + 2;
         int zero[2] = {0, 0};
         SED_HIP(hipMemcpyAsync(step_, zero, sizeof(zero), hipMemcpyHostToDevice, st_));
         SED_HIP(hipStreamSynchronize(st_));  // `zero` is a stack temporary
@@ -425,7 +457,6 @@ class ThickLanczos {
             int j = k;
             while (j < m_) {
                 int chunk = std::min(m_ - j, j < nev + 2 ? nev + 2 - j : kCheckEvery);
-                chunk = std::max(1, std::min(chunk, maxiter - out.iters));
                 for (int c = 0; c < chunk; ++c) SED_TRY(step(use_graph && out.iters > 0));
                 out.iters += chunk;
                 out.matvecs += chunk;
@@ -458,7 +489,7 @@ class ThickLanczos {
                     const double res = beta * std::fabs(Sm[(size_t)(jm - 1) * jm + i]);
                     if (res <= tol) ++conv; else break;
                 }
-                if (breakdown || conv >= nev || out.iters >= maxiter) {
+                if (breakdown || conv >= nev || (j == m_ && out.restarts >= maxiter)) {
                     done = true;
                     break;
                 }
@@ -508,14 +539,19 @@ class ThickLanczos {
     // one Lanczos step as plain launches
     int launch_step() {
         SED_TRY(launch_matvec_xhat(sec_, xh_, w_, VecTraits<T>::fp64, st_));
-        hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_);
-        hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h1_);
+        hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_,
+                           (const int*)nullptr);
+        hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h1_,
+                           (const int*)nullptr);
         hipLaunchKernelGGL(k_update<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h1_, w_, 0,
-                           normpart_);
-        hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_);
-        hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h2_);
+                           normpart_, (const int*)nullptr);
+        // second pass only when the first cancelled most of w (decided on the device)
+        hipLaunchKernelGGL(k_reorth_check, dim3(1), dim3(kRedBlock), 0, st_, step_, normpart_, kUpdGrid, h1_, h2_,
+                           gate_);
+        hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_, gate_);
+        hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h2_, gate_);
         hipLaunchKernelGGL(k_update<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h2_, w_, 1,
-                           normpart_);
+                           normpart_, gate_);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kRedBlock), 0, st_, step_, h1_, h2_, normpart_, kUpdGrid, alpha_,
                            beta_, scale_, flag_);
         hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n_)), dim3(256), 0, st_, V_, (int64_t)n_, step_, scale_,
@@ -612,7 +648,7 @@ class ThickLanczos {
     double* normpart_ = nullptr;
     double2 *h1_ = nullptr, *h2_ = nullptr;
     double *alpha_ = nullptr, *beta_ = nullptr, *scale_ = nullptr;
-    int *step_ = nullptr, *flag_ = nullptr;
+    int *step_ = nullptr, *flag_ = nullptr, *gate_ = nullptr;
 };
 
 int64_t solve_footprint(const sed_sector* sec, int nev, int precision) {
@@ -719,15 +755,21 @@ int solve_sector(sed_sector* sec, int nev, double tol, uint64_t seed, int maxite
             if (st.converged < nev) return finish(fail(SED_ENOCONV, "Lanczos (FP32) did not converge the requested eigenpairs"));
             return finish(SED_OK);
         }
-        // FP64 refinement restarted from the sum of the FP32 Ritz vectors
-        SED_TRY(grow(ws.v0, (size_t)2 * n));
-        double2* v0 = reinterpret_cast<double2*>(ws.v0.p);
-        hipLaunchKernelGGL(k_sum_vectors<float2>, dim3(grid_for(n)), dim3(256), 0, s, L32.V_, (int64_t)n, L32.nritz_,
-                           v0, (int64_t)n);
-        SED_HIP(hipGetLastError());
+        // FP64 refinement restarted from the sum of the FP32 Ritz vectors (from the seeded
+        // random vector when the wanted FP32 Ritz values are degenerate)
         ThickLanczos<double2> L64(sec, m, ws);
-        SED_TRY(L64.init());
-        SED_TRY(L64.start_from(v0));
+        if (fp32_cluster(o.evals, nev)) {
+            SED_TRY(L64.init());
+            SED_TRY(L64.start_random(seed));
+        } else {
+            SED_TRY(grow(ws.v0, (size_t)2 * n));
+            double2* v0 = reinterpret_cast<double2*>(ws.v0.p);
+            hipLaunchKernelGGL(k_sum_vectors<float2>, dim3(grid_for(n)), dim3(256), 0, s, L32.V_, (int64_t)n,
+                               L32.nritz_, v0, (int64_t)n);
+            SED_HIP(hipGetLastError());
+            SED_TRY(L64.init());
+            SED_TRY(L64.start_from(v0));
+        }
         SED_TRY(L64.run(nev, tol, maxiter, o));
         st.iters_fp64 = o.iters;
         if (out_vecs) {

@@ -34,7 +34,6 @@ constexpr int kPackBlock = 1024;
 constexpr int kPackCheck = 8;             // steps between host checks
 constexpr int64_t kPackMaxDim = 1 << 17;
 constexpr int kPackDots = 4;              // basis vectors per sweep of the dot products
-constexpr double kReorthEta2 = 0.01;      // second pass when |w'|^2 <= eta^2 |w|^2, eta = 0.1
 constexpr int kMoveRows = 64;             // rows per workgroup of the restart kernel
 constexpr int kSlotDoubles = 2 * kMaxBasis + 2;   // alpha[m], beta[m], unused, (int step, int flag)
 
@@ -393,13 +392,15 @@ struct PackReq {
     int start = kStartNone;  // pending start of the next launch
     int set_step = -1;       // pending restart step of the next launch
     int iters[2] = {0, 0};
+    int cycles[2] = {0, 0};  // thick restarts per precision phase
     int restarts = 0;
     std::chrono::steady_clock::time_point t0;
 };
 
 // What one check point decided for one request (pure host computation).
 struct Decision {
-    int action = 0;          // 0 continue, 1 done, 2 hand over FP32 -> FP64, 3 thick restart, -1 fail
+    int action = 0;          // 0 continue, 1 done, 2 hand over FP32 -> FP64, 3 thick restart,
+                             // 4 FP64 phase from the random start, -1 fail
     int rc = SED_OK;
     std::string err;
     int jm = 0, kk = 0;
@@ -557,6 +558,8 @@ class PackEngine {
                 d.action = -1;
                 d.rc = SED_ENOCONV;
                 d.err = "Lanczos (packed) did not converge the requested eigenpairs";
+            } else if (q.phase == 0 && job->precision == SED_PREC_FP32_REFINE && fp32_cluster(d.theta, job->nev)) {
+                d.action = 4;   // degenerate FP32 Ritz values: FP64 phase from the seeded random vector
             } else if (q.phase == 0 && job->precision == SED_PREC_FP32_REFINE) {
                 // FP64 start vector = sum of the wanted FP32 Ritz vectors
                 d.action = 2;
@@ -569,10 +572,10 @@ class PackEngine {
             }
             return;
         }
-        if (q.iters[q.phase] >= job->maxiter) {
+        if (jm == m && q.cycles[q.phase] >= job->maxiter) {
             d.action = -1;
             d.rc = SED_ENOCONV;
-            d.err = "Lanczos (packed) reached maxiter";
+            d.err = "Lanczos (packed) reached maxiter restarts";
             return;
         }
         if (jm == m) {
@@ -607,6 +610,13 @@ class PackEngine {
                 job->stats.residual_max = d.resmax;
                 finish(q, SED_OK, "");
                 return 1;
+            case 4:
+                q.phase = 1;
+                q.start = kStartRandom;
+                q.T.assign((size_t)q.m * q.m, 0.0);
+                q.k = 0;
+                q.j = 0;
+                return 0;
             case 2:
             case 3: {
                 PackMove mv{};
@@ -633,6 +643,7 @@ class PackEngine {
                     q.k = d.kk;
                     q.j = d.kk;
                     ++q.restarts;
+                    ++q.cycles[q.phase];
                 }
                 moves.push_back({mv, f32 ? 0 : 1});
                 return 0;

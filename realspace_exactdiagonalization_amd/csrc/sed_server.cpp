@@ -21,6 +21,7 @@ namespace sed {
 namespace {
 
 using clk = std::chrono::steady_clock;
+constexpr size_t kLaneKeepBytes = (size_t)1 << 30;   // idle lane workspace kept for reuse
 
 struct Request {
     int64_t id = 0;
@@ -60,6 +61,7 @@ struct sed_server {
     int64_t n_done = 0;
     int64_t n_submitted = 0;
     bool stop = false;
+    bool counted = false;   // registered as a child of ctx
 };
 
 namespace sed {
@@ -102,6 +104,8 @@ void lane_main(sed_server* srv, Lane* lane) {
         sed_solve_stats st{};
         const int rc = solve_sector(sec, nev, tol, seed, maxiter, prec, ev, nullptr, &st, lane->ws);
         const std::string err = rc == SED_OK ? std::string() : std::string(sed_last_error());
+        // the admission budget counts running requests only: give large idle buffers back
+        if (lane->ws.device_bytes() > kLaneKeepBytes) lane->ws.release();
         lk.lock();
         Request& r2 = srv->reqs[lane->job];
         r2.evals = ev;
@@ -178,6 +182,8 @@ int sed_server_create(sed_context* ctx, int64_t hbm_budget_bytes, int max_slots,
         }
     }
     *out = srv;
+    context_child_added(ctx);
+    srv->counted = true;
     return SED_OK;
 }
 
@@ -293,7 +299,10 @@ int sed_server_destroy(sed_server* srv) {
         lane->ws.release();
         delete lane;
     }
+    sed_context* ctx = srv->ctx;
+    const bool counted = srv->counted;
     delete srv;
+    if (counted) context_child_removed(ctx);
     return SED_OK;
 }
 

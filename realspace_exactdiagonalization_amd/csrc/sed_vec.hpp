@@ -10,6 +10,10 @@
 namespace sed {
 
 constexpr int kMaxBasis = 64;   // largest Krylov basis (m)
+// Second classical Gram-Schmidt pass only when the first one left |w'|^2 <= eta^2 |w|^2
+// (Daniel-Gragg-Kaufman-Stewart test, eta = 0.1): a single pass loses at most a factor
+// 1/eta of orthogonality relative to the vector precision.
+constexpr double kReorthEta2 = 0.01;
 
 // Krylov basis size for a sector of dimension n and nev wanted eigenpairs.
 inline int krylov_dim(int64_t n, int nev) {

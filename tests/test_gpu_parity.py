@@ -272,6 +272,45 @@ def test_lanczos_graph_replay_matches_eager(ed, ctx, monkeypatch):
         assert np.array_equal(a, b)
 
 
+def test_step_graph_not_reused_for_a_new_sector_of_equal_shape(ed, oracle_mod):
+    """Regression: sectors of equal dimension built one after another, each destroyed
+    (with its catalog) before the next is built, must not replay the previous sector's
+    captured Lanczos step.  Each solve matches the oracle's eigenvalues."""
+    c = ed.Context(0)
+    try:
+        for th in (0.0, 0.37, 0.0, 0.61):
+            m = L.twisted(L.heisenberg_chain(16), (th, 0.0))
+            G = L.model_group(m, (th, 0.0))
+            print(f"[graph-reuse] xxz16 theta={th}", flush=True)
+            ir = L.translation_irreps(G, m.lattice)[0]
+            oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, False)
+            os_ = oracle_mod.Sector(oc, ir.values)
+            os_.set_terms(*L.term_arrays(m))
+            ref = oracle_mod.lowest_eigenvalues(os_, 3)
+            gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                                 particle_statistics=m.statistics, ctx=c)
+            gb = ed.build_symmetry_sector_basis(gc, ir)
+            assert gb.dim == 810
+            H, n = ed.hamiltonian_linear_operator(gb, m.bilinear_terms, m.density_terms)
+            vals = ed.diagonalize_block_matrixfree(H, n, nev=3, return_vectors=False)[0]
+            assert np.abs(vals - ref).max() <= 1e-10 * np.abs(ref).max(), (th, vals, ref)
+            if th == 0.37:
+                gc.close()   # catalog first: released with its last sector
+                gb.close()
+            else:
+                gb.close()
+                gc.close()
+        # context first, then its sector and catalog
+        gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                             particle_statistics=m.statistics, ctx=c)
+        gb = ed.build_symmetry_sector_basis(gc, ir)
+        c.close()
+        gb.close()
+        gc.close()
+    finally:
+        c.close()
+
+
 def test_packed_engine_matches_lane_path(ed, ctx, monkeypatch):
     """Small sectors solved in the packed persistent launches give the eigenvalues of the
     per-sector lane path (FP64 and FP32+refine), for bosons and fermions."""
