@@ -27,7 +27,32 @@ PEAK_HBM_GBS = 8000.0
 
 
 def algorithmic_bytes(dim, gated, ins, s):
+    """Algorithmic bytes of one sector matvec with the per-unit figures of the workload
+    definition: per row the representative (8 B), its norm (4 B), x[row] and the y[row]
+    write (2s); per gated hop one rep-table entry of 8 B; per in-sector hop one gather of
+    x (s).  s = bytes per complex element."""
+    return dim * (8 + 4 + 2 * s) + gated * 8 + ins * s
+
+
+def layout_bytes(dim, gated, ins, s):
+    """The same count for this library's compressed layout: 1-byte stabilizer order and
+    a 4-byte table entry (row, group element and sign packed)."""
     return dim * (8 + 1 + 2 * s) + gated * 4 + ins * s
+
+
+def measured_traffic(precision):
+    """HBM bytes per launch of the sector matvec on this workload, from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, scaled by the plain-copy
+    calibration); None when no summary for the current kernel is present."""
+    path = os.path.join(ROOT, "profiles", "matvec_checker4x6_counters.md")
+    if not os.path.exists(path):
+        return None
+    key = f"traffic_bytes_{precision}:"
+    with open(path) as f:
+        for line in f:
+            if line.strip().startswith(key):
+                return int(line.split(":", 1)[1].strip())
+    return None
 
 
 def cpu_baseline(model, G, ir, cat, basis, seconds, threads, bytes_c64, gated, ins):
@@ -346,8 +371,9 @@ def main():
         ms = H.benchmark(prec, args.steps)
         wall = time.time() - w0
         B = algorithmic_bytes(dim, gated, ins, s)
+        Bl = layout_bytes(dim, gated, ins, s)
         res[name] = {"ms": ms, "bytes": B, "gbs": B / (ms * 1e-3) / 1e9, "wall_s": wall,
-                     "shape": basis.shape(prec)}
+                     "layout_bytes": Bl, "layout_gbs": Bl / (ms * 1e-3) / 1e9, "shape": basis.shape(prec)}
         print(f"[bench] {name}: {ms:.3f} ms/matvec  {B/1e9:.2f} GB algorithmic  "
               f"{res[name]['gbs']:.0f} GB/s ({100*res[name]['gbs']/PEAK_HBM_GBS:.1f}% of 8 TB/s)  "
               f"shape {res[name]['shape']}", flush=True)
@@ -388,10 +414,14 @@ def main():
                    "n_site": model.n_site, "n_filled": model.n_filled, "group_order": G.order,
                    "gated_hops": gated, "in_sector_hops": ins},
         "roofline": {"bound": "hbm", "achieved": round(main_r["gbs"], 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(main_r["gbs"] / PEAK_HBM_GBS, 4), "traffic": None},
+                     "frac": round(main_r["gbs"] / PEAK_HBM_GBS, 4), "traffic": measured_traffic("c32")},
+        "algorithmic_bytes_per_launch": main_r["bytes"],
+        "layout_bytes": {"c32": res["c32"]["layout_bytes"], "c64": res["c64"]["layout_bytes"],
+                         "c32_GB/s": round(res["c32"]["layout_gbs"], 2), "c64_GB/s": round(res["c64"]["layout_gbs"], 2)},
         "cpu_baseline": cpu,
         "fp64": {"ms_per_matvec": round(res["c64"]["ms"], 4), "GB/s": round(res["c64"]["gbs"], 2),
-                 "frac": round(res["c64"]["gbs"] / PEAK_HBM_GBS, 4), "launch_shape": res["c64"]["shape"]},
+                 "frac": round(res["c64"]["gbs"] / PEAK_HBM_GBS, 4), "traffic": measured_traffic("c64"),
+                 "launch_shape": res["c64"]["shape"]},
         "launch_shape": res["c32"]["shape"],
         "flux_scan_haldane3x4": flux,
         "serving_stream": stream,

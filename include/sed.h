@@ -121,14 +121,21 @@ int sed_matvec_device(sed_sector* sec, const void* x_dev, void* y_dev, int preci
 int sed_matvec_benchmark(sed_sector* sec, int precision, int n_calls, double* ms_per_call);
 /* Launch shape of the H|psi> kernel (no reference counterpart; the reference has one
  * threaded loop).  Shapes are indexed 0..n_shapes-1; each has `lanes` lanes per row
- * (64/lanes rows per wavefront) and `batch` hops in flight per lane.  shape = -1
- * re-enables the autotuner, which times every shape on the first matvec of a sector
- * of dimension >= 2^17 and keeps the fastest (smaller sectors use shape 1). */
-int sed_sector_set_shape(sed_sector* sec, int precision, int shape);
-/* Chosen shape (-1 if not chosen yet), its lanes/batch, the number of shapes, and the
- * autotuner's mean time per shape in ms (n_shapes floats, 0 if not timed; may be NULL). */
-int sed_sector_shape(const sed_sector* sec, int precision, int* shape, int* lanes, int* batch, int* n_shapes,
-                     float* tune_ms);
+ * (64/lanes rows per wavefront) and `batch` hops in flight per lane.  xcd_chunk is the
+ * run length in which each of the 8 XCDs takes consecutive row blocks (1 = natural
+ * order, 0 = one contiguous range per XCD).  shape = -1 re-enables the autotuner
+ * (xcd_chunk ignored), which on the first matvec of a sector of dimension >= 2^17 times
+ * every shape, then the run lengths for the fastest one, then the shapes again at the
+ * fastest run length, and keeps the fastest pair (smaller sectors use shape 1, run
+ * length 1). */
+int sed_sector_set_shape(sed_sector* sec, int precision, int shape, int xcd_chunk);
+/* Chosen shape (-1 if not chosen yet), its lanes/batch and XCD run length, the numbers
+ * of shapes and of run lengths tried, and the autotuner's mean times in ms (may be
+ * NULL; 2 * n_shapes + n_chunks floats: every shape at run length 1, every run length
+ * for the fastest of those, every shape again at the fastest run length; 0 if not
+ * timed). */
+int sed_sector_shape(const sed_sector* sec, int precision, int* shape, int* lanes, int* batch, int* xcd_chunk,
+                     int* n_shapes, int* n_chunks, float* tune_ms);
 /* diagonalize_block_matrixfree(H_op, n; nev) (:998-1030): lowest nev eigenvalues
  * (ascending) and optionally eigenvectors (sector_dim x nev column-major ComplexF64,
  * may be NULL).  Sectors of dim <= 500 are solved densely, as the reference does;

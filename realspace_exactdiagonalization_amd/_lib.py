@@ -77,9 +77,9 @@ SIGNATURES = {
     "sed_matvec": (C.c_int, [_vp, _f64p, _f64p, C.c_int]),
     "sed_matvec_device": (C.c_int, [_vp, _vp, _vp, C.c_int]),
     "sed_matvec_benchmark": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
-    "sed_sector_set_shape": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "sed_sector_set_shape": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "sed_sector_shape": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                   C.POINTER(C.c_int), _vp]),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
     "sed_solve": (C.c_int, [_vp, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int, _f64p, _vp,
                             C.POINTER(SolveStats)]),
     "sed_sector_density": (C.c_int, [_vp, C.c_int, _f64p]),

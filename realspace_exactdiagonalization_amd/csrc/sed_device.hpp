@@ -84,13 +84,15 @@ struct C2<double> { using T = double2; };
 template <>
 struct C2<float> { using T = float2; };
 
-// Grid-block index remapped so that each XCD (blocks are dealt round-robin over
-// the 8 XCDs) works on one contiguous range of logical blocks; returns -1 for the
-// padding blocks of a grid rounded up to a multiple of 8.
-__device__ __forceinline__ int64_t xcd_block(int64_t nblocks_logical) {
+// Logical block of this workgroup.  Blocks are dealt round-robin to the 8 XCDs
+// (physical block b runs on XCD b % 8); here XCD x takes the logical blocks in runs of
+// `chunk`: chunk = 1 keeps the natural order (all XCDs on neighbouring rows at once,
+// sharing the MALL), chunk = ceil(n / 8) gives each XCD one contiguous range (sharing
+// its own L2 only).  The grid is a multiple of 8 * chunk; padding blocks get -1.
+__device__ __forceinline__ int64_t xcd_block(int64_t nblocks_logical, int64_t chunk) {
     const int64_t b = blockIdx.x;
-    const int64_t per = (nblocks_logical + 7) / 8;
-    const int64_t lb = (b & 7) * per + (b >> 3);
+    const int64_t x = b & 7, s = b >> 3;
+    const int64_t lb = (s / chunk) * (8 * chunk) + x * chunk + (s % chunk);
     return lb < nblocks_logical ? lb : -1;
 }
 

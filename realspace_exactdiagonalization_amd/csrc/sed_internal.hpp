@@ -120,8 +120,9 @@ struct sed_sector {
     sed::DevBuf<double> d_gauge;      // F(rep_r) per row, complex interleaved (only if has_gauge)
     int64_t hops_gated = -1, hops_in_sector = -1;
     int mv_shape[2] = {-1, -1};       // H|psi> launch shape per precision (FP64, FP32), -1 = not tuned
+    int mv_chunk[2] = {1, 1};         // XCD run length of the H|psi> blocks (0 = one range per XCD)
     int64_t version = 0;              // bumped whenever device term tables are (re)uploaded
-    float tune_ms[2][8] = {};         // autotuner timings per shape
+    float tune_ms[2][32] = {};        // autotuner timings: shapes, XCD run lengths, shapes again
     int busy = 0;                     // a server request is running on this sector
     // last solve
     std::vector<double> last_evals;
@@ -138,11 +139,12 @@ namespace sed {
 struct GraphKey {
     const void* sec = nullptr;
     int64_t version = -1;
-    int prec = -1, m = 0, shape = -1;
+    int prec = -1, m = 0, shape = -1, chunk = -1;
     int64_t n = 0;
     const void* ptrs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     bool operator==(const GraphKey& o) const {
-        if (sec != o.sec || version != o.version || prec != o.prec || m != o.m || shape != o.shape || n != o.n)
+        if (sec != o.sec || version != o.version || prec != o.prec || m != o.m || shape != o.shape ||
+            chunk != o.chunk || n != o.n)
             return false;
         for (int i = 0; i < 5; ++i)
             if (ptrs[i] != o.ptrs[i]) return false;

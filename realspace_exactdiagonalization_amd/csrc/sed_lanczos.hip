@@ -570,6 +570,7 @@ class ThickLanczos {
         key.prec = VecTraits<T>::fp64 ? 0 : 1;
         key.m = m_;
         key.shape = sec_->mv_shape[key.prec];
+        key.chunk = sec_->mv_chunk[key.prec];
         key.n = n_;
         key.ptrs[0] = V_;
         key.ptrs[1] = w_;

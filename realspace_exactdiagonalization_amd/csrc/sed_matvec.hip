@@ -49,11 +49,12 @@ __global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* 
                                                      const uint8_t* __restrict__ stab,
                                                      const uint32_t* __restrict__ table,
                                                      const typename C2<R>::T* __restrict__ xhat,
-                                                     typename C2<R>::T* __restrict__ y, int64_t nblocks) {
+                                                     typename C2<R>::T* __restrict__ y, int64_t nblocks,
+                                                     int64_t chunk) {
     using T = typename C2<R>::T;
     constexpr int RPB = kMvBlock / LPR;
     const MvLds<R> L = stage_mv<R>(c, kMvBlock);
-    const int64_t lb = xcd_block(nblocks);
+    const int64_t lb = xcd_block(nblocks, chunk);
     if (lb < 0) return;
     const int sub = threadIdx.x % LPR;
     const int64_t r = lb * RPB + threadIdx.x / LPR;
@@ -87,6 +88,9 @@ struct MvShape {
 };
 static const MvShape kShapes[] = {{1, 1}, {1, 4}, {2, 4}, {4, 4}, {8, 4}, {4, 2}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+// XCD run lengths the autotuner tries for the chosen shape (0 = one range per XCD)
+static const int kChunks[] = {1, 16, 64, 128, 256, 512, 1024, 4096, 0};
+constexpr int kNumChunks = sizeof(kChunks) / sizeof(kChunks[0]);
 
 // Dense H' (row-major, dim x dim, complex double) for the small-sector path.
 template <typename M, bool FERMI>
@@ -267,48 +271,66 @@ template size_t matvec_lds_bytes<double>(const sed_sector*);
 
 template <typename M, typename R, bool F, int LPR, int B>
 static int launch_shape(const MvConst& c, size_t lds, sed_sector* sec, const typename C2<R>::T* xh,
-                        typename C2<R>::T* y, hipStream_t s) {
+                        typename C2<R>::T* y, int chunk, hipStream_t s) {
     const int64_t rpb = kMvBlock / LPR;
     const int64_t nb = (sec->dim + rpb - 1) / rpb;
-    const int64_t grid = ((nb + 7) / 8) * 8;
+    const int64_t ch = chunk > 0 ? chunk : std::max<int64_t>(1, (nb + 7) / 8);
+    const int64_t grid = ((nb + 8 * ch - 1) / (8 * ch)) * (8 * ch);
     hipLaunchKernelGGL((k_matvec<M, R, F, LPR, B>), dim3((unsigned)grid), dim3(kMvBlock), lds, s, c, sec->d_rep.p,
-                       sec->d_stab.p, sec->d_table.p, xh, y, nb);
+                       sec->d_stab.p, sec->d_table.p, xh, y, nb, ch);
     SED_HIP(hipGetLastError());
     return SED_OK;
 }
 
 template <typename M, typename R, bool F>
-static int run_shape(sed_sector* sec, int shape, const typename C2<R>::T* xh, typename C2<R>::T* y,
+static int run_shape(sed_sector* sec, int shape, int chunk, const typename C2<R>::T* xh, typename C2<R>::T* y,
                      hipStream_t s) {
     MvConst c = make_const<R>(sec);
     size_t off[9];
     const size_t lds = mv_lds_layout<R>(c, off, kMvBlock);
     switch (shape) {
-        case 0: return launch_shape<M, R, F, 1, 1>(c, lds, sec, xh, y, s);
-        case 1: return launch_shape<M, R, F, 1, 4>(c, lds, sec, xh, y, s);
-        case 2: return launch_shape<M, R, F, 2, 4>(c, lds, sec, xh, y, s);
-        case 3: return launch_shape<M, R, F, 4, 4>(c, lds, sec, xh, y, s);
-        case 4: return launch_shape<M, R, F, 8, 4>(c, lds, sec, xh, y, s);
-        default: return launch_shape<M, R, F, 4, 2>(c, lds, sec, xh, y, s);
+        case 0: return launch_shape<M, R, F, 1, 1>(c, lds, sec, xh, y, chunk, s);
+        case 1: return launch_shape<M, R, F, 1, 4>(c, lds, sec, xh, y, chunk, s);
+        case 2: return launch_shape<M, R, F, 2, 4>(c, lds, sec, xh, y, chunk, s);
+        case 3: return launch_shape<M, R, F, 4, 4>(c, lds, sec, xh, y, chunk, s);
+        case 4: return launch_shape<M, R, F, 8, 4>(c, lds, sec, xh, y, chunk, s);
+        default: return launch_shape<M, R, F, 4, 2>(c, lds, sec, xh, y, chunk, s);
     }
 }
 
 template <typename R>
-static int run_any_shape(sed_sector* sec, int shape, const typename C2<R>::T* xh, typename C2<R>::T* y,
+static int run_any_shape(sed_sector* sec, int shape, int chunk, const typename C2<R>::T* xh, typename C2<R>::T* y,
                          hipStream_t s) {
     const bool F = sec->cat->stats == SED_FERMIONIC;
     if (sec->cat->n_site <= 32)
-        return F ? run_shape<uint32_t, R, true>(sec, shape, xh, y, s) : run_shape<uint32_t, R, false>(sec, shape, xh, y, s);
-    return F ? run_shape<uint64_t, R, true>(sec, shape, xh, y, s) : run_shape<uint64_t, R, false>(sec, shape, xh, y, s);
+        return F ? run_shape<uint32_t, R, true>(sec, shape, chunk, xh, y, s)
+                 : run_shape<uint32_t, R, false>(sec, shape, chunk, xh, y, s);
+    return F ? run_shape<uint64_t, R, true>(sec, shape, chunk, xh, y, s)
+             : run_shape<uint64_t, R, false>(sec, shape, chunk, xh, y, s);
 }
 
 // Autotuner: on the first matvec of a sector large enough for the launch shape to
-// matter, time every shape on the call's own buffers (deterministic kernels, so
-// the repeated launches leave y as one launch would) and keep the fastest.  A
-// process-wide cache keyed by the sector's shape parameters skips re-tuning.
+// matter, time every shape (XCD run length 1) on the call's own buffers, then every XCD
+// run length for the fastest shape (deterministic kernels, so the repeated launches
+// leave y as one launch would), and keep the fastest pair.  A process-wide cache keyed
+// by the sector's shape parameters skips re-tuning.
 static std::mutex g_tune_mu;
-static std::map<std::tuple<int, int, int, int, int64_t, int>, int> g_tune_cache;
+static std::map<std::tuple<int, int, int, int, int64_t, int>, std::pair<int, int>> g_tune_cache;
 constexpr int64_t kTuneMinDim = 1 << 17;
+
+template <typename R>
+static int time_shape(sed_sector* sec, int shape, int chunk, const typename C2<R>::T* xh, typename C2<R>::T* y,
+                      hipStream_t s, hipEvent_t e0, hipEvent_t e1, float* ms_out) {
+    SED_TRY(run_any_shape<R>(sec, shape, chunk, xh, y, s));  // warm-up
+    SED_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < 3; ++k) SED_TRY(run_any_shape<R>(sec, shape, chunk, xh, y, s));
+    SED_HIP(hipEventRecord(e1, s));
+    SED_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SED_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *ms_out = ms / 3;
+    return SED_OK;
+}
 
 template <typename R>
 static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<R>::T* y, hipStream_t s) {
@@ -318,28 +340,51 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<
         const int v = std::atoi(env);
         if (v >= 0 && v < kNumShapes) return sec->mv_shape[pi] = v;
     }
-    if (sec->dim < kTuneMinDim) return sec->mv_shape[pi] = 1;
+    if (sec->dim < kTuneMinDim) {
+        sec->mv_chunk[pi] = 1;
+        return sec->mv_shape[pi] = 1;
+    }
     const auto key = std::make_tuple(sec->cat->n_site, sec->cat->n_filled, sec->cat->n_group, sec->n_terms,
                                      sec->dim, pi);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(key);
-        if (it != g_tune_cache.end()) return sec->mv_shape[pi] = it->second;
+        if (it != g_tune_cache.end()) {
+            sec->mv_chunk[pi] = it->second.second;
+            return sec->mv_shape[pi] = it->second.first;
+        }
     }
     hipEvent_t e0, e1;
     SED_HIP(hipEventCreate(&e0));
     SED_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    int best_i = 1;
+    int best_i = 1, best_c = 1;
     for (int i = 0; i < kNumShapes; ++i) {
-        SED_TRY(run_any_shape<R>(sec, i, xh, y, s));  // warm-up
-        SED_HIP(hipEventRecord(e0, s));
-        for (int k = 0; k < 3; ++k) SED_TRY(run_any_shape<R>(sec, i, xh, y, s));
-        SED_HIP(hipEventRecord(e1, s));
-        SED_HIP(hipEventSynchronize(e1));
         float ms = 0;
-        SED_HIP(hipEventElapsedTime(&ms, e0, e1));
-        sec->tune_ms[pi][i] = ms / 3;
+        SED_TRY(time_shape<R>(sec, i, 1, xh, y, s, e0, e1, &ms));
+        sec->tune_ms[pi][i] = ms;
+        if (ms < best) {
+            best = ms;
+            best_i = i;
+        }
+    }
+    for (int c = 0; c < kNumChunks; ++c) {
+        float ms = 0;
+        if (kChunks[c] == 1) ms = sec->tune_ms[pi][best_i];
+        else SED_TRY(time_shape<R>(sec, best_i, kChunks[c], xh, y, s, e0, e1, &ms));
+        sec->tune_ms[pi][kNumShapes + c] = ms;
+        if (ms < best) {
+            best = ms;
+            best_c = kChunks[c];
+        }
+    }
+    // the shapes once more at the chosen run length
+    const int shape_c = best_i;
+    for (int i = 0; i < kNumShapes; ++i) {
+        float ms = 0;
+        if (i == shape_c) ms = sec->tune_ms[pi][kNumShapes + (int)(std::find(kChunks, kChunks + kNumChunks, best_c) - kChunks)];
+        else SED_TRY(time_shape<R>(sec, i, best_c, xh, y, s, e0, e1, &ms));
+        sec->tune_ms[pi][kNumShapes + kNumChunks + i] = ms;
         if (ms < best) {
             best = ms;
             best_i = i;
@@ -349,8 +394,9 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<
     (void)hipEventDestroy(e1);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune_cache[key] = best_i;
+        g_tune_cache[key] = {best_i, best_c};
     }
+    sec->mv_chunk[pi] = best_c;
     return sec->mv_shape[pi] = best_i;
 }
 
@@ -358,16 +404,19 @@ template <typename R>
 static int run_mv(sed_sector* sec, const typename C2<R>::T* xh, typename C2<R>::T* y, hipStream_t s) {
     const int shape = pick_shape<R>(sec, xh, y, s);
     if (shape < 0) return shape;
-    return run_any_shape<R>(sec, shape, xh, y, s);
+    return run_any_shape<R>(sec, shape, sec->mv_chunk[sizeof(R) == 8 ? 0 : 1], xh, y, s);
 }
 
-int matvec_shape_info(const sed_sector* sec, int precision, int* lanes, int* batch, float* ms_per_shape) {
+int matvec_shape_info(const sed_sector* sec, int precision, int* lanes, int* batch, int* chunk, float* tune_ms,
+                      int* n_chunks) {
     const int pi = precision == SED_PREC_FP64 ? 0 : 1;
     const int sh = sec->mv_shape[pi];
     if (lanes) *lanes = sh >= 0 ? kShapes[sh].lanes : 0;
     if (batch) *batch = sh >= 0 ? kShapes[sh].batch : 0;
-    if (ms_per_shape)
-        for (int i = 0; i < kNumShapes; ++i) ms_per_shape[i] = sec->tune_ms[pi][i];
+    if (chunk) *chunk = sec->mv_chunk[pi];
+    if (n_chunks) *n_chunks = kNumChunks;
+    if (tune_ms)
+        for (int i = 0; i < 2 * kNumShapes + kNumChunks; ++i) tune_ms[i] = sec->tune_ms[pi][i];
     return kNumShapes;
 }
 

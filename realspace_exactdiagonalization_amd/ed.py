@@ -175,16 +175,19 @@ class SymmetrySectorBasis:
                                          np.zeros(2)))
         self._terms_set = True
 
-    def set_shape(self, precision: int, shape: int):
-        """Force H|psi> launch shape `shape` (-1: autotune)."""
-        check(lib().sed_sector_set_shape(self._h, precision, shape))
+    def set_shape(self, precision: int, shape: int, xcd_chunk: int = 1):
+        """Force H|psi> launch shape `shape` and XCD run length (shape -1: autotune)."""
+        check(lib().sed_sector_set_shape(self._h, precision, shape, xcd_chunk))
 
     def shape(self, precision: int) -> dict:
-        sh, ln, bt, ns = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        ms = np.zeros(16, dtype=np.float32)
-        check(lib().sed_sector_shape(self._h, precision, C.byref(sh), C.byref(ln), C.byref(bt), C.byref(ns), ptr(ms)))
-        return {"shape": sh.value, "lanes_per_row": ln.value, "batch": bt.value,
-                "tune_ms": [round(float(v), 4) for v in ms[:ns.value]]}
+        sh, ln, bt, ch, ns, nc = (C.c_int() for _ in range(6))
+        ms = np.zeros(32, dtype=np.float32)
+        check(lib().sed_sector_shape(self._h, precision, C.byref(sh), C.byref(ln), C.byref(bt), C.byref(ch),
+                                     C.byref(ns), C.byref(nc), ptr(ms)))
+        return {"shape": sh.value, "lanes_per_row": ln.value, "batch": bt.value, "xcd_chunk": ch.value,
+                "tune_ms": [round(float(v), 4) for v in ms[:ns.value]],
+                "tune_ms_xcd_chunk": [round(float(v), 4) for v in ms[ns.value:ns.value + nc.value]],
+                "tune_ms_at_chunk": [round(float(v), 4) for v in ms[ns.value + nc.value:2 * ns.value + nc.value]]}
 
     def hop_counts(self) -> Tuple[int, int]:
         g, s = C.c_int64(), C.c_int64()

@@ -241,9 +241,9 @@ def test_every_launch_shape_matches_oracle(ed, ctx, oracle_mod):
     n_shapes = len(gb.shape(ed.PREC_FP64)["tune_ms"])
     assert n_shapes >= 4
     for prec, tol in ((ed.PREC_FP64, 1e-12), (ed.PREC_FP32, 5e-6)):
-        for sh in range(n_shapes):
-            print(f"[shape] prec={prec} shape={sh}", flush=True)
-            gb.set_shape(prec, sh)
+        for sh, chunk in [(s, 1) for s in range(n_shapes)] + [(2, 0), (2, 3), (1, 32)]:
+            print(f"[shape] prec={prec} shape={sh} xcd_chunk={chunk}", flush=True)
+            gb.set_shape(prec, sh, chunk)
             H.precision = prec
             y1, y2 = H(x), H(x)
             assert np.array_equal(y1, y2)
