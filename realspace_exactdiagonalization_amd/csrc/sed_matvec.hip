@@ -337,8 +337,13 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<
     const int pi = sizeof(R) == 8 ? 0 : 1;
     if (sec->mv_shape[pi] >= 0) return sec->mv_shape[pi];
     if (const char* env = std::getenv("SED_MATVEC_SHAPE")) {
+        // fixed launch (profiling): shape from the environment, run length SED_MATVEC_CHUNK
         const int v = std::atoi(env);
-        if (v >= 0 && v < kNumShapes) return sec->mv_shape[pi] = v;
+        const char* ce = std::getenv("SED_MATVEC_CHUNK");
+        if (v >= 0 && v < kNumShapes) {
+            sec->mv_chunk[pi] = ce ? std::max(0, std::atoi(ce)) : 1;
+            return sec->mv_shape[pi] = v;
+        }
     }
     if (sec->dim < kTuneMinDim) {
         sec->mv_chunk[pi] = 1;
