@@ -89,7 +89,7 @@ struct MvShape {
 static const MvShape kShapes[] = {{1, 1}, {1, 4}, {2, 4}, {4, 4}, {8, 4}, {4, 2}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // XCD run lengths the autotuner tries for the chosen shape (0 = one range per XCD)
-static const int kChunks[] = {1, 16, 64, 128, 256, 512, 1024, 4096, 0};
+static const int kChunks[] = {1, 16, 64, 256, 1024, 0};
 constexpr int kNumChunks = sizeof(kChunks) / sizeof(kChunks[0]);
 
 // Dense H' (row-major, dim x dim, complex double) for the small-sector path.
@@ -311,24 +311,31 @@ static int run_any_shape(sed_sector* sec, int shape, int chunk, const typename C
 
 // Autotuner: on the first matvec of a sector large enough for the launch shape to
 // matter, time every shape (XCD run length 1) on the call's own buffers, then every XCD
-// run length for the fastest shape (deterministic kernels, so the repeated launches
-// leave y as one launch would), and keep the fastest pair.  A process-wide cache keyed
-// by the sector's shape parameters skips re-tuning.
+// run length for the fastest shape, then every shape at the fastest run length
+// (deterministic kernels, so the repeated launches leave y as one launch would), and
+// keep the fastest pair.  Each candidate's time is the minimum over three separate
+// launches, so work of other streams running meanwhile inflates it less.  A
+// process-wide cache keyed by the model's shape parameters and the sector dimension
+// rounded to a quarter octave skips re-tuning (all momentum sectors of a model share it).
 static std::mutex g_tune_mu;
-static std::map<std::tuple<int, int, int, int, int64_t, int>, std::pair<int, int>> g_tune_cache;
+static std::map<std::tuple<int, int, int, int, int, int, int>, std::pair<int, int>> g_tune_cache;
 constexpr int64_t kTuneMinDim = 1 << 17;
 
 template <typename R>
 static int time_shape(sed_sector* sec, int shape, int chunk, const typename C2<R>::T* xh, typename C2<R>::T* y,
                       hipStream_t s, hipEvent_t e0, hipEvent_t e1, float* ms_out) {
     SED_TRY(run_any_shape<R>(sec, shape, chunk, xh, y, s));  // warm-up
-    SED_HIP(hipEventRecord(e0, s));
-    for (int k = 0; k < 3; ++k) SED_TRY(run_any_shape<R>(sec, shape, chunk, xh, y, s));
-    SED_HIP(hipEventRecord(e1, s));
-    SED_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    SED_HIP(hipEventElapsedTime(&ms, e0, e1));
-    *ms_out = ms / 3;
+    float best = 1e30f;
+    for (int k = 0; k < 3; ++k) {
+        SED_HIP(hipEventRecord(e0, s));
+        SED_TRY(run_any_shape<R>(sec, shape, chunk, xh, y, s));
+        SED_HIP(hipEventRecord(e1, s));
+        SED_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SED_HIP(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms);
+    }
+    *ms_out = best;
     return SED_OK;
 }
 
@@ -349,8 +356,8 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<
         sec->mv_chunk[pi] = 1;
         return sec->mv_shape[pi] = 1;
     }
-    const auto key = std::make_tuple(sec->cat->n_site, sec->cat->n_filled, sec->cat->n_group, sec->n_terms,
-                                     sec->dim, pi);
+    const auto key = std::make_tuple(sec->cat->n_site, sec->cat->n_filled, sec->cat->n_group, sec->cat->stats,
+                                     sec->n_terms, (int)std::lround(4.0 * std::log2((double)sec->dim)), pi);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(key);

@@ -32,7 +32,7 @@ MvConst mv_const(const sed_sector* sec, bool fp64);
 
 constexpr int kPackBlock = 1024;
 constexpr int kPackCheck = 8;             // steps between host checks
-constexpr int64_t kPackMaxDim = 1 << 17;
+constexpr int64_t kPackMaxDim = 1 << 15;   // one CU per sector: larger sectors go to the lanes
 constexpr int kPackDots = 4;              // basis vectors per sweep of the dot products
 constexpr int kMoveRows = 64;             // rows per workgroup of the restart kernel
 constexpr int kSlotDoubles = 2 * kMaxBasis + 2;   // alpha[m], beta[m], unused, (int step, int flag)

@@ -205,8 +205,12 @@ int sed_server_submit(sed_server* srv, sed_sector* sec, int nev, double tol, uin
     r.precision = precision;
     r.t_submit = clk::now();
     ++srv->n_submitted;
-    if (srv->pack && pack_eligible(sec, nev)) {
-        // small sector: joins the packed persistent launches at the next check point
+    // Small sectors take a free lane when one is idle (lowest latency); when every lane is
+    // busy they join the packed persistent launches at the next check point (throughput).
+    bool lane_idle = false;
+    if (srv->queue.empty())
+        for (const Lane* l : srv->lanes) lane_idle = lane_idle || l->job < 0;
+    if (srv->pack && !lane_idle && pack_eligible(sec, nev)) {
         r.status = 2;
         r.t_start = r.t_submit;
         r.footprint = pack_footprint(sec, nev, precision);
