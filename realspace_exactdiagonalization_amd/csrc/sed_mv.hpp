@@ -137,44 +137,42 @@ __device__ __forceinline__ void row_sums(const MvLds<R>& L, const MvConst& c, co
                 dy += L.dv[d].y;
             }
         }
+        // BATCH hops at a time, without branches: every candidate computes its rank; the
+        // gated-out ones (target occupied, or past the site's list) read entry 0 of the
+        // table and of xhat (valid addresses, cache-resident) and contribute zero, so the
+        // BATCH table reads, then the BATCH gathers, are issued back to back.
         const int t0 = L.tstart[a], t1 = L.tstart[a + 1];
         for (int t = t0; t < t1; t += BATCH) {
-            uint32_t e[BATCH];
+            uint32_t idx[BATCH];
+            bool ok[BATCH];
             int pj[BATCH];
 #pragma unroll
             for (int u = 0; u < BATCH; ++u) {
-                e[u] = kNoRow;
-                pj[u] = 0;
-                const int tt = t + u;
-                if (tt < t1) {
-                    const int b = L.tto[tt];
-                    if (!((m >> b) & 1)) {
-                        const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
-                        e[u] = table[S[ib * SST] + L.bin[b * c.bs + ib + 1]];
-                        if (FERMI) pj[u] = (ib - ia) & 1;
-                    }
-                }
+                const int tt = min(t + u, t1 - 1);
+                const int b = L.tto[tt];
+                const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
+                ok[u] = (t + u < t1) && !((m >> b) & 1);
+                idx[u] = S[ib * SST] + L.bin[b * c.bs + ib + 1];
+                pj[u] = FERMI ? ((ib - ia) & 1) : 0;
+            }
+            uint32_t e[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const uint32_t v = table[ok[u] ? idx[u] : 0u];
+                e[u] = ok[u] ? v : kNoRow;
             }
             T v[BATCH];
 #pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
-                v[u].x = 0;
-                v[u].y = 0;
-                if (e[u] != kNoRow) v[u] = xhat[e[u] >> c.shift];
-            }
+            for (int u = 0; u < BATCH; ++u) v[u] = xhat[e[u] != kNoRow ? (e[u] >> c.shift) : 0u];
 #pragma unroll
             for (int u = 0; u < BATCH; ++u) {
-                if (e[u] == kNoRow) continue;
-                const int g = (e[u] >> 1) & c.gmask;
-                const T cf = cmul(L.tcoef[t + u], L.chi[g]);
+                const bool live = e[u] != kNoRow;
+                const int g = live ? (int)((e[u] >> 1) & c.gmask) : 0;
+                const T cf = cmul(L.tcoef[min(t + u, t1 - 1)], L.chi[g]);
                 const T pr = cmul(cf, v[u]);
-                if (((e[u] & 1) ^ pj[u]) != 0) {
-                    ax -= pr.x;
-                    ay -= pr.y;
-                } else {
-                    ax += pr.x;
-                    ay += pr.y;
-                }
+                const R sg = live ? ((((e[u] & 1) ^ (uint32_t)pj[u]) != 0) ? (R)-1 : (R)1) : (R)0;
+                ax += sg * pr.x;
+                ay += sg * pr.y;
             }
         }
         ia += LPR;
