@@ -24,6 +24,7 @@
 #include <cstring>
 
 #include "sed_device.hpp"
+#include "sed_gemm.hpp"
 #include "sed_internal.hpp"
 #include "sed_vec.hpp"
 
@@ -32,7 +33,7 @@ namespace sed {
 int launch_dense(sed_sector* sec, double* H_dev, hipStream_t s);
 int launch_matvec_xhat(sed_sector* sec, const void* xhat, void* y, bool fp64, hipStream_t s);
 
-constexpr int kRedGrid = 512;     // fixed reduction grid (determinism)
+constexpr int kRedGrid = 1024;    // fixed reduction grid (determinism)
 constexpr int kRedBlock = 256;
 constexpr int kDotGroup = 8;
 constexpr int kRitzRows = 32;
@@ -391,6 +392,16 @@ struct SolveOut {
     int64_t matvecs = 0;
 };
 
+// Re-orthogonalisation and restart rotation on the FP64 matrix cores (default), or as
+// row-wise VALU loops with SED_REORTH=valu (kept for comparison).
+static bool reorth_mfma() {
+    static const bool on = [] {
+        const char* e = std::getenv("SED_REORTH");
+        return !(e && std::strcmp(e, "valu") == 0);
+    }();
+    return on;
+}
+
 static bool graphs_enabled() {
     const char* e = std::getenv("SED_LANCZOS_GRAPH");
     return !(e && e[0] == '0');
@@ -539,23 +550,37 @@ class ThickLanczos {
     // one Lanczos step as plain launches
     int launch_step() {
         SED_TRY(launch_matvec_xhat(sec_, xh_, w_, VecTraits<T>::fp64, st_));
-        hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_,
-                           (const int*)nullptr);
-        hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h1_,
-                           (const int*)nullptr);
-        hipLaunchKernelGGL(k_update<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h1_, w_, 0,
-                           normpart_, (const int*)nullptr);
-        // second pass only when the first cancelled most of w (decided on the device)
+        // first classical Gram-Schmidt pass; the second only when the first cancelled most
+        // of w (decided on the device)
+        SED_TRY(launch_orth_pass(h1_, 0, nullptr));
         hipLaunchKernelGGL(k_reorth_check, dim3(1), dim3(kRedBlock), 0, st_, step_, normpart_, kUpdGrid, h1_, h2_,
                            gate_);
-        hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_, gate_);
-        hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h2_, gate_);
-        hipLaunchKernelGGL(k_update<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h2_, w_, 1,
-                           normpart_, gate_);
+        SED_TRY(launch_orth_pass(h2_, 1, gate_));
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kRedBlock), 0, st_, step_, h1_, h2_, normpart_, kUpdGrid, alpha_,
                            beta_, scale_, flag_);
         hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n_)), dim3(256), 0, st_, V_, (int64_t)n_, step_, scale_,
                            sec_->d_stab.p, xh_);
+        SED_HIP(hipGetLastError());
+        return SED_OK;
+    }
+
+    // One Gram-Schmidt pass: h = V[0..j]^H w (and |w|^2), then w - V h into w
+    // (to_next = 0) or into v_{j+1} (to_next = 1), with the norm partials of the result.
+    int launch_orth_pass(double2* h, int to_next, const int* gate) {
+        if (reorth_mfma()) {
+            hipLaunchKernelGGL(k_proj_mfma<T>, dim3(G_), dim3(256), 0, st_, V_, (int64_t)n_, step_, w_, part_,
+                               gate);
+        } else {
+            hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_, gate);
+        }
+        hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h, gate);
+        if (reorth_mfma()) {
+            hipLaunchKernelGGL(k_orth_update_mfma<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h, w_,
+                               to_next, normpart_, gate);
+        } else {
+            hipLaunchKernelGGL(k_update<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h, w_, to_next,
+                               normpart_, gate);
+        }
         SED_HIP(hipGetLastError());
         return SED_OK;
     }
@@ -621,10 +646,16 @@ class ThickLanczos {
     }
     int ritz(int m, const std::vector<double>& S, int kk) {
         SED_HIP(hipMemcpyAsync(ws_.S.p, S.data(), S.size() * sizeof(double), hipMemcpyHostToDevice, st_));
-        const size_t lds = (((size_t)m * kk * 8 + 15) & ~(size_t)15) + (size_t)m * kRitzRows * sizeof(T);
-        const unsigned g = (unsigned)((n_ + kRitzRows - 1) / kRitzRows);
-        hipLaunchKernelGGL(k_ritz<T>, dim3(g), dim3(kRitzRows), lds, st_, V_, (int64_t)n_, m, ws_.S.p, kk,
-                           (int64_t)n_);
+        if (reorth_mfma()) {
+            const unsigned g = (unsigned)std::min<int64_t>(4096, (n_ + 31) / 32);
+            hipLaunchKernelGGL(k_rotate_mfma<T>, dim3(g), dim3(256), 0, st_, V_, (int64_t)n_, m, ws_.S.p, kk,
+                               (int64_t)n_);
+        } else {
+            const size_t lds = (((size_t)m * kk * 8 + 15) & ~(size_t)15) + (size_t)m * kRitzRows * sizeof(T);
+            const unsigned g = (unsigned)((n_ + kRitzRows - 1) / kRitzRows);
+            hipLaunchKernelGGL(k_ritz<T>, dim3(g), dim3(kRitzRows), lds, st_, V_, (int64_t)n_, m, ws_.S.p, kk,
+                               (int64_t)n_);
+        }
         SED_HIP(hipGetLastError());
         SED_HIP(hipStreamSynchronize(st_));  // S is host-pageable
         return SED_OK;
