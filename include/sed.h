@@ -148,6 +148,16 @@ int sed_solve(sed_sector* sec, int nev, double tol, uint64_t seed, int maxiter, 
 /* Site densities <n_i> of eigenvector `which` (0-based) of the last sed_solve on this
  * sector, n_site doubles (observables/static_structure_factor.jl:62-83 convention). */
 int sed_sector_density(sed_sector* sec, int which, double* density);
+/* Density-density correlation of eigenvector `which` of the last sed_solve:
+ * nn[i * n_site + j] = <n_i n_j> (0-based sites, i == j included) and
+ * density[i] = <n_i>, formed as the orbit average of
+ * _build_density_correlation_matrix! (observables/static_structure_factor.jl:41-104)
+ * before its disconnected part is subtracted. */
+int sed_sector_density_correlation(sed_sector* sec, int which, double* nn, double* density);
+/* One-body density matrix rho[i * n_site + j] = <a_i^dag a_j> of eigenvector `which`
+ * of the last sed_solve (complex interleaved, 0-based sites), the quantity of
+ * off_diagonal_long_range_order (observables/off_diagonal_long_range_order.jl:55-126). */
+int sed_sector_one_body(sed_sector* sec, int which, double* rho);
 int sed_sector_destroy(sed_sector* sec);
 
 /* Continuous-batching service of many sector solves (no reference counterpart; it

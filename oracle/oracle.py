@@ -87,6 +87,8 @@ def lib():
         L.oc_apply_table.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int]
         L.oc_apply_hamiltonian.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _i32p,
                                            _f64p, _f64p, _f64p, C.c_int, C.c_int64, C.c_int64]
+        L.oc_density_correlation.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
+        L.oc_one_body.argtypes = [C.c_void_p, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -286,6 +288,26 @@ class Sector:
 
     def dense(self) -> np.ndarray:
         return self.sparse().toarray()
+
+    def density_correlation(self, c):
+        """(<n_i n_j> as an n_site x n_site matrix, <n_i>) of the sector state c, as
+        _build_density_correlation_matrix! forms them (static_structure_factor.jl:41-104)
+        before the disconnected part is subtracted."""
+        n = self.catalog.n_site
+        c = np.ascontiguousarray(c, dtype=np.complex128)
+        C = np.zeros(n * n)
+        d = np.zeros(n)
+        lib().oc_density_correlation(self._h, c.view(np.float64), C, d)
+        return C.reshape(n, n), d
+
+    def one_body(self, c):
+        """rho[i, j] = <psi| a_i^dag a_j |psi> of the sector state c
+        (off_diagonal_long_range_order.jl:74-126)."""
+        n = self.catalog.n_site
+        c = np.ascontiguousarray(c, dtype=np.complex128)
+        rho = np.zeros(n * n, dtype=np.complex128)
+        lib().oc_one_body(self._h, c.view(np.float64), rho.view(np.float64))
+        return rho.reshape(n, n)
 
 
 def sector_for(model, group, chi) -> Sector:

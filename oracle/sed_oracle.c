@@ -635,3 +635,75 @@ void oc_apply_hamiltonian(const oc_sector* s, int n_bil, const int32_t* ifrom, c
     }
     free(buf); free(hop); free(from);
 }
+
+/* ------------------------------------------------------------------ */
+/* sector observables (observables/ directory)                         */
+/* ------------------------------------------------------------------ */
+
+/* _build_density_correlation_matrix! (observables/static_structure_factor.jl:41-104)
+ * with every site in both flavours and the disconnected part kept: for each
+ * representative with amplitude c_col, each of the |G| images g(rep) adds
+ * |c_col|^2/|G| to density[i] for its occupied i and to C[i][j] (row-major, 0-based)
+ * for its occupied pairs (i, j), i == j included. */
+void oc_density_correlation(const oc_sector* s, const double* c_, double* C, double* density) {
+    const cplx* c = (const cplx*)c_;
+    const oc_catalog* cat = s->cat;
+    const int n = cat->G.n_site, nG = cat->G.n_group;
+    for (int i = 0; i < n * n; ++i) C[i] = 0.0;
+    for (int i = 0; i < n; ++i) density[i] = 0.0;
+    const double nG_inv = 1.0 / nG;
+    for (int64_t col = 0; col < s->dim; ++col) {
+        const double w = creal(c[col]) * creal(c[col]) + cimag(c[col]) * cimag(c[col]);
+        if (w == 0.0) continue;
+        const double w_eff = w * nG_inv;
+        for (int g = 0; g < nG; ++g) {
+            cplx a;
+            const mask_t shifted = apply_op(s->reps[col], &cat->G, g, &a);
+            for (mask_t t = shifted; t; t &= t - 1) density[__builtin_ctzll(t)] += w_eff;
+            for (mask_t t2 = shifted; t2; t2 &= t2 - 1) {
+                const int i = __builtin_ctzll(t2);
+                for (mask_t t3 = shifted; t3; t3 &= t3 - 1) C[i * n + __builtin_ctzll(t3)] += w_eff;
+            }
+        }
+    }
+}
+
+/* off_diagonal_long_range_order (observables/off_diagonal_long_range_order.jl:74-126):
+ * rho[i][j] = <psi| a_i^dag a_j |psi> (row-major, 0-based, complex interleaved) by the
+ * orbit-basis contraction: every image U_g|rep> of every representative, every
+ * occupied j and empty i, the scattered mask projected back onto the sector. */
+void oc_one_body(const oc_sector* s, const double* c_, double* rho_) {
+    const cplx* c = (const cplx*)c_;
+    cplx* rho = (cplx*)rho_;
+    const oc_catalog* cat = s->cat;
+    const int n = cat->G.n_site, nG = cat->G.n_group;
+    for (int i = 0; i < n * n; ++i) rho[i] = 0.0;
+    const double inv_nG = 1.0 / nG, inv_sqrt_nG = 1.0 / sqrt((double)nG);
+    for (int64_t col = 0; col < s->dim; ++col) {
+        const cplx c_col = c[col];
+        if (c_col == 0) continue;
+        const double diag_weight = (creal(c_col) * creal(c_col) + cimag(c_col) * cimag(c_col)) * inv_nG;
+        const double ket_norm = inv_sqrt_nG / sqrt((double)s->stab[col]);
+        for (int g = 0; g < nG; ++g) {
+            cplx ket_group_phase;
+            const mask_t ket = apply_op(s->reps[col], &cat->G, g, &ket_group_phase);
+            const cplx ket_basis_amp = (ket_norm * conj(s->chi[g])) * ket_group_phase;
+            const cplx ket_amp = c_col * ket_basis_amp;
+            for (mask_t tj = ket; tj; tj &= tj - 1) {
+                const int j = __builtin_ctzll(tj) + 1;
+                rho[(j - 1) * n + (j - 1)] += diag_weight;
+                for (int i = 1; i <= n; ++i) {
+                    if (i == j || ((ket >> (i - 1)) & 1)) continue;
+                    const mask_t nm = (ket & ~((mask_t)1 << (j - 1))) | ((mask_t)1 << (i - 1));
+                    cplx proj_coeff;
+                    const int64_t row = project(s, nm, &proj_coeff);
+                    if (row == 0) continue;
+                    const cplx bra_basis_amp = (sqrt((double)s->stab[row - 1]) * inv_sqrt_nG) * conj(proj_coeff);
+                    const cplx bra_amp = c[row - 1] * bra_basis_amp;
+                    const double phase = hop_phase(cat->G.fermionic, ket, j, i);
+                    rho[(i - 1) * n + (j - 1)] += (conj(bra_amp) * ket_amp) * phase;
+                }
+            }
+        }
+    }
+}

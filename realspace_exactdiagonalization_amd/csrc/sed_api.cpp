@@ -24,6 +24,9 @@ int launch_gauge_rows(sed_sector* sec, const double* f_dev);
 template <typename R>
 size_t matvec_lds_bytes(const sed_sector* sec);
 int launch_density(sed_sector* sec, const double2* vec, double* density);
+int launch_pair_density(sed_sector* sec, const double2* vec, std::vector<double>& M);
+int launch_cdot(const double2* x, const double2* y, int64_t n, double2* part, std::vector<double2>& host,
+                hipStream_t s, cd* out);
 int launch_convert(const double* a, float* b, int64_t n, hipStream_t s);
 int launch_convert(const float* a, double* b, int64_t n, hipStream_t s);
 int launch_matvec_xhat(sed_sector* sec, const void* xhat, void* y, bool fp64, hipStream_t s);
@@ -646,6 +649,137 @@ int sed_sector_density(sed_sector* sec, int which, double* density) {
     SED_HIP(hipMemcpy(v.p, sec->last_evecs.data() + (size_t)which * sec->dim, (size_t)16 * sec->dim,
                       hipMemcpyHostToDevice));
     return launch_density(sec, v.p, density);
+}
+
+// the requested eigenvector of the last solve, uploaded (reference basis)
+static int upload_eigvec(sed_sector* sec, int which, DevBuf<double2>& v) {
+    if (which < 0 || which >= sec->last_nev) return fail(SED_ESTATE, "no eigenvector " + std::to_string(which));
+    SED_TRY(v.alloc(sec->dim));
+    SED_HIP(hipMemcpy(v.p, sec->last_evecs.data() + (size_t)which * sec->dim, (size_t)16 * sec->dim,
+                      hipMemcpyHostToDevice));
+    return SED_OK;
+}
+
+int sed_sector_density_correlation(sed_sector* sec, int which, double* nn, double* density) {
+    if (!sec || !nn || !density) return fail(SED_EINVAL, "NULL argument");
+    SED_HIP(hipSetDevice(sec->cat->ctx->device));
+    DevBuf<double2> v;
+    SED_TRY(upload_eigvec(sec, which, v));
+    std::vector<double> M;
+    const int rc = launch_pair_density(sec, v.p, M);
+    v.release();
+    SED_TRY(rc);
+    // each image g(rep) occupies site i iff rep occupies pinv_g(i), perm_g(pinv_g(i)) = i:
+    // <n_i n_j> = (1/|G|) sum_g M[pinv_g(i)][pinv_g(j)]
+    const sed_catalog* cat = sec->cat;
+    const int n = cat->n_site, nG = cat->n_group;
+    std::vector<int> pinv((size_t)nG * n);
+    for (int g = 0; g < nG; ++g)
+        for (int a = 0; a < n; ++a) pinv[(size_t)g * n + (cat->perm[(size_t)g * n + a] - 1)] = a;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double sm = 0.0;
+            for (int g = 0; g < nG; ++g) sm += M[(size_t)pinv[(size_t)g * n + i] * n + pinv[(size_t)g * n + j]];
+            nn[(size_t)i * n + j] = sm / nG;
+        }
+    for (int i = 0; i < n; ++i) density[i] = nn[(size_t)i * n + i];
+    return SED_OK;
+}
+
+// rho_ij = <psi| a_i^dag a_j |psi>.  A sector state is an eigenvector of every U_g, so
+// rho_ij = <psi| O_ij |psi> with O_ij = (1/|G|) sum_g U_g a_i^dag a_j U_g^dag
+//        = (1/|G|) sum_g eta_g(i) conj(eta_g(j)) a_{g(i)}^dag a_{g(j)}
+// (U_g a_i^dag U_g^dag = eta_g(i) a_{g(i)}^dag with the group's site phases), a
+// translation-invariant hopping operator: the sector H|psi> kernel applies it.  The
+// kernel gathers rows through Hermiticity, so it is fed the Hermitian parts
+// A = O_ij + O_ji and B = i (O_ij - O_ji): rho_ij = (<A> - i <B>) / 2, rho_ji = conj.
+int sed_sector_one_body(sed_sector* sec, int which, double* rho) {
+    if (!sec || !rho) return fail(SED_EINVAL, "NULL argument");
+    sed_catalog* cat = sec->cat;
+    SED_HIP(hipSetDevice(cat->ctx->device));
+    const int n = cat->n_site, nG = cat->n_group;
+    const int64_t dim = sec->dim;
+    hipStream_t s = cat->ctx->stream;
+    DevBuf<double2> x;
+    SED_TRY(upload_eigvec(sec, which, x));
+    std::vector<double> dens(n);
+    SED_TRY(launch_density(sec, x.p, dens.data()));
+    for (int i = 0; i < 2 * n * n; ++i) rho[i] = 0.0;
+    for (int i = 0; i < n; ++i) rho[2 * ((size_t)i * n + i)] = dens[i];
+    SED_TRY(launch_gauge64(sec, reinterpret_cast<double*>(x.p), true, s));   // primed basis
+    // a scratch sector on this sector's tables with the pair operators as its terms
+    sed_sector tmp;
+    tmp.cat = cat;
+    tmp.chi = sec->chi;
+    tmp.dim = dim;
+    tmp.shift = sec->shift;
+    tmp.d_rep = sec->d_rep;
+    tmp.d_stab = sec->d_stab;
+    tmp.d_table = sec->d_table;
+    tmp.d_chi64 = sec->d_chi64;
+    tmp.d_chi32 = sec->d_chi32;
+    tmp.has_gauge = sec->has_gauge;
+    tmp.d_gauge = sec->d_gauge;
+    tmp.mv_shape[0] = sec->mv_shape[0] >= 0 ? sec->mv_shape[0] : 1;
+    tmp.mv_chunk[0] = sec->mv_shape[0] >= 0 ? sec->mv_chunk[0] : 1;
+    DevBuf<double2> y, xh, part;
+    int rc = y.alloc(dim);
+    if (rc == SED_OK) rc = xh.alloc(dim);
+    if (rc == SED_OK) rc = part.alloc(256);
+    std::vector<int32_t> from(2 * nG), to(2 * nG);
+    std::vector<double> t(4 * nG);
+    std::vector<double2> host;
+    for (int i = 0; i < n && rc == SED_OK; ++i)
+        for (int j = i + 1; j < n && rc == SED_OK; ++j) {
+            double ev[2] = {0.0, 0.0};
+            for (int part_b = 0; part_b < 2 && rc == SED_OK; ++part_b) {
+                for (int g = 0; g < nG; ++g) {
+                    const cd tg = cat->phases[(size_t)g * n + i] * std::conj(cat->phases[(size_t)g * n + j]) /
+                                  (double)nG;
+                    const cd f = part_b ? cd(0.0, 1.0) * tg : tg;   // O_ij part; the O_ji part is its conjugate
+                    from[2 * g] = cat->perm[(size_t)g * n + j];
+                    to[2 * g] = cat->perm[(size_t)g * n + i];
+                    from[2 * g + 1] = cat->perm[(size_t)g * n + i];
+                    to[2 * g + 1] = cat->perm[(size_t)g * n + j];
+                    t[4 * g] = f.real();
+                    t[4 * g + 1] = f.imag();
+                    t[4 * g + 2] = f.real();
+                    t[4 * g + 3] = -f.imag();
+                }
+                rc = sed_sector_set_terms(&tmp, 2 * nG, from.data(), to.data(), t.data(), 0, nullptr, nullptr,
+                                          nullptr);
+                if (rc == SED_OK)
+                    rc = launch_matvec64(&tmp, reinterpret_cast<const double*>(x.p), reinterpret_cast<double*>(y.p),
+                                         reinterpret_cast<double*>(xh.p), s);
+                cd dot = 0.0;
+                if (rc == SED_OK) rc = launch_cdot(x.p, y.p, dim, part.p, host, s, &dot);
+                ev[part_b] = dot.real();
+            }
+            const cd r(0.5 * ev[0], -0.5 * ev[1]);   // (<A> - i <B>) / 2
+            rho[2 * ((size_t)i * n + j)] = r.real();
+            rho[2 * ((size_t)i * n + j) + 1] = r.imag();
+            rho[2 * ((size_t)j * n + i)] = r.real();
+            rho[2 * ((size_t)j * n + i) + 1] = -r.imag();
+        }
+    // the scratch sector borrowed these tables: hand them back before it goes away
+    tmp.d_rep = DevBuf<uint64_t>();
+    tmp.d_stab = DevBuf<uint8_t>();
+    tmp.d_table = DevBuf<uint32_t>();
+    tmp.d_chi64 = DevBuf<double>();
+    tmp.d_chi32 = DevBuf<float>();
+    tmp.d_gauge = DevBuf<double>();
+    for (auto* b : {&tmp.d_term_start, &tmp.d_dens_start}) b->release();
+    tmp.d_term_to.release();
+    tmp.d_dens_j.release();
+    tmp.d_term_coef64.release();
+    tmp.d_dens_v64.release();
+    tmp.d_term_coef32.release();
+    tmp.d_dens_v32.release();
+    y.release();
+    xh.release();
+    part.release();
+    x.release();
+    return rc;
 }
 
 int sed_sector_destroy(sed_sector* sec) {

@@ -139,12 +139,12 @@ namespace sed {
 struct GraphKey {
     const void* sec = nullptr;
     int64_t version = -1;
-    int prec = -1, m = 0, shape = -1, chunk = -1;
+    int prec = -1, m = 0, shape = -1, chunk = -1, reorth = -1;
     int64_t n = 0;
     const void* ptrs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     bool operator==(const GraphKey& o) const {
         if (sec != o.sec || version != o.version || prec != o.prec || m != o.m || shape != o.shape ||
-            chunk != o.chunk || n != o.n)
+            chunk != o.chunk || reorth != o.reorth || n != o.n)
             return false;
         for (int i = 0; i < 5; ++i)
             if (ptrs[i] != o.ptrs[i]) return false;

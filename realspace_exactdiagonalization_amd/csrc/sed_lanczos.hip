@@ -393,13 +393,10 @@ struct SolveOut {
 };
 
 // Re-orthogonalisation and restart rotation on the FP64 matrix cores (default), or as
-// row-wise VALU loops with SED_REORTH=valu (kept for comparison).
+// row-wise VALU loops with SED_REORTH=valu (kept for comparison; read once per solve).
 static bool reorth_mfma() {
-    static const bool on = [] {
-        const char* e = std::getenv("SED_REORTH");
-        return !(e && std::strcmp(e, "valu") == 0);
-    }();
-    return on;
+    const char* e = std::getenv("SED_REORTH");
+    return !(e && std::strcmp(e, "valu") == 0);
 }
 
 static bool graphs_enabled() {
@@ -412,7 +409,8 @@ static bool graphs_enabled() {
 template <typename T>
 class ThickLanczos {
    public:
-    ThickLanczos(sed_sector* sec, int m, Workspace& ws) : sec_(sec), n_(sec->dim), m_(m), ws_(ws), st_(ws.stream) {}
+    ThickLanczos(sed_sector* sec, int m, Workspace& ws)
+        : sec_(sec), n_(sec->dim), m_(m), ws_(ws), st_(ws.stream), mfma_(reorth_mfma()) {}
 
     int init() {
         SED_TRY(VecTraits<T>::reserve(ws_, n_, m_));
@@ -567,14 +565,14 @@ class ThickLanczos {
     // One Gram-Schmidt pass: h = V[0..j]^H w (and |w|^2), then w - V h into w
     // (to_next = 0) or into v_{j+1} (to_next = 1), with the norm partials of the result.
     int launch_orth_pass(double2* h, int to_next, const int* gate) {
-        if (reorth_mfma()) {
+        if (mfma_) {
             hipLaunchKernelGGL(k_proj_mfma<T>, dim3(G_), dim3(256), 0, st_, V_, (int64_t)n_, step_, w_, part_,
                                gate);
         } else {
             hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_, gate);
         }
         hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h, gate);
-        if (reorth_mfma()) {
+        if (mfma_) {
             hipLaunchKernelGGL(k_orth_update_mfma<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h, w_,
                                to_next, normpart_, gate);
         } else {
@@ -596,6 +594,7 @@ class ThickLanczos {
         key.m = m_;
         key.shape = sec_->mv_shape[key.prec];
         key.chunk = sec_->mv_chunk[key.prec];
+        key.reorth = mfma_ ? 1 : 0;
         key.n = n_;
         key.ptrs[0] = V_;
         key.ptrs[1] = w_;
@@ -646,7 +645,7 @@ class ThickLanczos {
     }
     int ritz(int m, const std::vector<double>& S, int kk) {
         SED_HIP(hipMemcpyAsync(ws_.S.p, S.data(), S.size() * sizeof(double), hipMemcpyHostToDevice, st_));
-        if (reorth_mfma()) {
+        if (mfma_) {
             const unsigned g = (unsigned)std::min<int64_t>(4096, (n_ + 31) / 32);
             hipLaunchKernelGGL(k_rotate_mfma<T>, dim3(g), dim3(256), 0, st_, V_, (int64_t)n_, m, ws_.S.p, kk,
                                (int64_t)n_);
@@ -681,6 +680,7 @@ class ThickLanczos {
     double2 *h1_ = nullptr, *h2_ = nullptr;
     double *alpha_ = nullptr, *beta_ = nullptr, *scale_ = nullptr;
     int *step_ = nullptr, *flag_ = nullptr, *gate_ = nullptr;
+    bool mfma_ = true;
 };
 
 int64_t solve_footprint(const sed_sector* sec, int nev, int precision) {

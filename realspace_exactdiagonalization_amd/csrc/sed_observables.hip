@@ -72,6 +72,133 @@ int launch_density(sed_sector* sec, const double2* vec, double* density) {
     return SED_OK;
 }
 
+// Pair occupations of the representatives: M[a][b] = sum_r |c_r|^2 n_a(rep_r) n_b(rep_r).
+// Each block takes a contiguous range of rows, staged 256 at a time in LDS; thread t owns
+// the entries t + 256 q of the n x n matrix and sums them in row order (no atomics), so
+// the per-block partials and their fixed-order sum are bitwise reproducible.
+constexpr int kPairOwn = 16;   // 16 x 256 entries = a 64 x 64 matrix
+__global__ void __launch_bounds__(kObsBlock) k_pair_density(const uint64_t* __restrict__ rep,
+                                                            const double2* __restrict__ c, int64_t dim, int n,
+                                                            double* __restrict__ partial) {
+    __shared__ uint64_t lm[kObsBlock];
+    __shared__ double lw[kObsBlock];
+    const int nn = n * n;
+    const int64_t chunk = (dim + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = (int64_t)blockIdx.x * chunk;
+    const int64_t r1 = std::min<int64_t>(dim, r0 + chunk);
+    int ea[kPairOwn], eb[kPairOwn];
+    double acc[kPairOwn];
+#pragma unroll
+    for (int q = 0; q < kPairOwn; ++q) {
+        const int e = threadIdx.x + q * kObsBlock;
+        ea[q] = e < nn ? e / n : 0;
+        eb[q] = e < nn ? e % n : 0;
+        acc[q] = 0.0;
+    }
+    for (int64_t t0 = r0; t0 < r1; t0 += kObsBlock) {
+        const int64_t r = t0 + threadIdx.x;
+        uint64_t m = 0;
+        double w = 0.0;
+        if (r < r1) {
+            m = rep[r];
+            const double2 v = c[r];
+            w = v.x * v.x + v.y * v.y;
+        }
+        __syncthreads();
+        lm[threadIdx.x] = m;
+        lw[threadIdx.x] = w;
+        __syncthreads();
+        const int cnt = (int)std::min<int64_t>(kObsBlock, r1 - t0);
+        for (int q2 = 0; q2 < cnt; ++q2) {
+            const uint64_t ms = lm[q2];
+            const double ws = lw[q2];
+#pragma unroll
+            for (int q = 0; q < kPairOwn; ++q) acc[q] += ((ms >> ea[q]) & (ms >> eb[q]) & 1) ? ws : 0.0;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kPairOwn; ++q) {
+        const int e = threadIdx.x + q * kObsBlock;
+        if (e < nn) partial[(int64_t)blockIdx.x * nn + e] = acc[q];
+    }
+}
+
+// out[e] = sum_b partial[b * nn + e], blocks in order
+__global__ void __launch_bounds__(kObsBlock) k_sum_partials(const double* __restrict__ partial, int nb, int nn,
+                                                            double* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nn) return;
+    double sm = 0.0;
+    for (int b = 0; b < nb; ++b) sm += partial[(int64_t)b * nn + e];
+    out[e] = sm;
+}
+
+int launch_pair_density(sed_sector* sec, const double2* vec, std::vector<double>& M) {
+    const sed_catalog* cat = sec->cat;
+    hipStream_t s = cat->ctx->stream;
+    const int n = cat->n_site, nn = n * n;
+    M.assign((size_t)nn, 0.0);
+    if (sec->dim == 0) return SED_OK;
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (sec->dim + kObsBlock - 1) / kObsBlock));
+    DevBuf<double> part, out;
+    SED_TRY(part.alloc((size_t)nb * nn));
+    SED_TRY(out.alloc((size_t)nn));
+    hipLaunchKernelGGL(k_pair_density, dim3((unsigned)nb), dim3(kObsBlock), 0, s, sec->d_rep.p, vec, sec->dim, n,
+                       part.p);
+    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((nn + kObsBlock - 1) / kObsBlock)), dim3(kObsBlock), 0, s,
+                       part.p, nb, nn, out.p);
+    SED_HIP(hipGetLastError());
+    SED_HIP(hipMemcpyAsync(M.data(), out.p, (size_t)nn * 8, hipMemcpyDeviceToHost, s));
+    SED_HIP(hipStreamSynchronize(s));
+    part.release();
+    out.release();
+    return SED_OK;
+}
+
+// partial[b] = sum over block b's rows of conj(x[r]) y[r] (fixed grid and order)
+__global__ void __launch_bounds__(kObsBlock) k_cdot(const double2* __restrict__ x, const double2* __restrict__ y,
+                                                    int64_t n, double2* __restrict__ partial) {
+    __shared__ double2 red[kObsBlock / kWave];
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = (int64_t)blockIdx.x * chunk;
+    const int64_t r1 = std::min<int64_t>(n, r0 + chunk);
+    double2 a = make_double2(0.0, 0.0);
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kObsBlock) {
+        const double2 u = x[r], v = y[r];
+        a.x += u.x * v.x + u.y * v.y;
+        a.y += u.x * v.y - u.y * v.x;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a.x += __shfl_down(a.x, off, kWave);
+        a.y += __shfl_down(a.y, off, kWave);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double2 sm = make_double2(0.0, 0.0);
+        for (int k = 0; k < kObsBlock / kWave; ++k) {
+            sm.x += red[k].x;
+            sm.y += red[k].y;
+        }
+        partial[blockIdx.x] = sm;
+    }
+}
+
+// <x|y> for complex double vectors on the device (part: at least kDotBlocks entries)
+int launch_cdot(const double2* x, const double2* y, int64_t n, double2* part, std::vector<double2>& host,
+                hipStream_t s, cd* out) {
+    constexpr int kDotBlocks = 256;
+    host.resize(kDotBlocks);
+    hipLaunchKernelGGL(k_cdot, dim3(kDotBlocks), dim3(kObsBlock), 0, s, x, y, n, part);
+    SED_HIP(hipGetLastError());
+    SED_HIP(hipMemcpyAsync(host.data(), part, kDotBlocks * sizeof(double2), hipMemcpyDeviceToHost, s));
+    SED_HIP(hipStreamSynchronize(s));
+    cd sm = 0.0;
+    for (const double2& v : host) sm += cd(v.x, v.y);
+    *out = sm;
+    return SED_OK;
+}
+
 __global__ void __launch_bounds__(256) k_d2f(const double2* __restrict__ a, float2* __restrict__ b, int64_t n) {
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
         b[r] = make_float2((float)a[r].x, (float)a[r].y);

@@ -31,7 +31,8 @@ __all__ = [
     "Context", "SymmetryOrbitCatalog", "SymmetrySectorBasis", "MatrixFreeHamiltonian",
     "build_symmetry_orbit_catalog", "build_symmetry_sector_basis", "hamiltonian_linear_operator",
     "diagonalize_block_matrixfree", "build_ed_data", "ed_scan_at_irrep_matrixfree", "ed_scan_flux",
-    "SectorServer", "PREC_FP64", "PREC_FP32", "PREC_FP32_REFINE",
+    "SectorServer", "PREC_FP64", "PREC_FP32", "PREC_FP32_REFINE", "sector_density",
+    "density_correlation_matrix", "off_diagonal_long_range_order",
 ]
 
 _default_ctx = None
@@ -267,6 +268,30 @@ def sector_density(H_op: MatrixFreeHamiltonian, which: int = 0) -> np.ndarray:
     d = np.zeros(H_op.basis.catalog.n_site, dtype=np.float64)
     check(lib().sed_sector_density(H_op.basis._h, which, d))
     return d
+
+
+def density_correlation_matrix(H_op: MatrixFreeHamiltonian, which: int = 0, subtract_disconnected: bool = True):
+    """``(C, density)`` of eigenvector ``which`` of the last solve: C[i, j] = <n_i n_j>
+    minus <n_i><n_j> when ``subtract_disconnected`` (0-based sites), as
+    ``_build_density_correlation_matrix!`` (observables/static_structure_factor.jl:41-113)
+    forms them with every site in both flavours."""
+    n = H_op.basis.catalog.n_site
+    nn = np.zeros(n * n, dtype=np.float64)
+    d = np.zeros(n, dtype=np.float64)
+    check(lib().sed_sector_density_correlation(H_op.basis._h, which, nn, d))
+    C = nn.reshape(n, n)
+    if subtract_disconnected:
+        C = C - np.outer(d, d)
+    return C, d
+
+
+def off_diagonal_long_range_order(H_op: MatrixFreeHamiltonian, which: int = 0) -> np.ndarray:
+    """One-body density matrix rho[i, j] = <a_i^dag a_j> (0-based sites) of eigenvector
+    ``which`` of the last solve (observables/off_diagonal_long_range_order.jl:55-126)."""
+    n = H_op.basis.catalog.n_site
+    rho = np.zeros(n * n, dtype=np.complex128)
+    check(lib().sed_sector_one_body(H_op.basis._h, which, rho.view(np.float64)))
+    return rho.reshape(n, n)
 
 
 @dataclass

@@ -277,3 +277,83 @@ def test_project_to_sector_agrees_with_unnormalized(oracle_mod):
             assert (a is None) == (z is None)
             if a is not None:
                 assert a[0] == z[0] and abs(a[1] - z[1]) < 1e-11
+
+
+# ---------------------------------------------------------------- observables
+def _fock_ops(model):
+    """Full fixed-N Fock basis (sorted occupation tuples) and a literal a^dag_i a_j on it."""
+    N, k = model.n_site, model.n_filled
+    states = list(itertools.combinations(range(N), k))
+    index = {s: i for i, s in enumerate(states)}
+    ferm = model.statistics == L.FERMIONIC
+
+    def hop(vec, i, j):
+        """a^dag_i a_j vec (0-based sites), signs from operator positions."""
+        out = np.zeros_like(vec)
+        for col, s in enumerate(states):
+            if vec[col] == 0 or j not in s or (i in s and i != j):
+                continue
+            lst = list(s)
+            p = lst.index(j)
+            sign = (-1) ** p if ferm else 1
+            lst.pop(p)
+            q = sum(1 for x in lst if x < i)
+            sign *= (-1) ** q if ferm else 1
+            lst.insert(q, i)
+            out[index[tuple(lst)]] += sign * vec[col]
+        return out
+
+    return states, index, hop
+
+
+def _embedding(oracle_mod, cat, sec, states, index, group, chi):
+    """Fock-space columns of the sector basis, |[s];chi> = sum_g conj(chi(g)) U_g|s> /
+    sqrt(|G| |Stab(s)|) (observables/entanglement_spectrum.jl:46-79)."""
+    reps, stab = sec.basis()
+    B = np.zeros((len(states), len(reps)), dtype=np.complex128)
+    for col, (r, so) in enumerate(zip(reps, stab)):
+        for g in range(group.order):
+            m, a = cat.apply_operation(int(r), g + 1)
+            occ = tuple(b for b in range(cat.n_site) if (m >> b) & 1)
+            B[index[occ], col] += np.conj(chi[g]) * a / math.sqrt(group.order * so)
+    return B
+
+
+OBS_CASES = {
+    "checker3x2_ferm_flux": (lambda: L.checkerboard(3, 2, n_filled=4, V1=1.0, V2=0.5), (0.23, 0.17), 2),
+    "hub2x3": (lambda: L.hubbard_square(2, 3, U=4.0, n_filled=5), (0.31, 0.0), 1),
+    "hald2x3_bos": (lambda: L.haldane_honeycomb(2, 3, n_filled=3, V1=0.3), (0.4, 0.0), 3),
+}
+
+
+@pytest.mark.parametrize("name", list(OBS_CASES))
+def test_sector_observables_against_fock_space(oracle_mod, name):
+    """<n_i n_j>, <n_i> and <a^dag_i a_j> of a sector eigenvector, as the reference's
+    orbit-basis loops form them, against literal operators on the state embedded in the
+    full Fock space.  The embedding is itself checked: orthonormal columns and
+    H_full B = B H_sector (a wrong sign or phase in the group action fails there)."""
+    f, th, k_idx = OBS_CASES[name]
+    m = L.twisted(f(), th)
+    G = L.model_group(m, th)
+    ir = L.translation_irreps(G, m.lattice)[k_idx]
+    cat, sec = _sector(oracle_mod, m, G, ir.values)
+    Hs = sec.dense()
+    w, v = np.linalg.eigh(Hs)
+    c = v[:, 0]
+    states, index, hop = _fock_ops(m)
+    B = _embedding(oracle_mod, cat, sec, states, index, G, ir.values)
+    Hf = full_fock_hamiltonian(m)
+    assert np.abs(B.conj().T @ B - np.eye(B.shape[1])).max() < 1e-12
+    assert np.abs(Hf @ B - B @ Hs).max() < 1e-11
+    psi = B @ c
+    N = m.n_site
+    C, d = sec.density_correlation(c)
+    occ = np.array([[1.0 if i in s else 0.0 for i in range(N)] for s in states])
+    p = np.abs(psi) ** 2
+    assert np.abs(d - occ.T @ p).max() < 1e-12
+    assert np.abs(C - (occ * p[:, None]).T @ occ).max() < 1e-12
+    assert abs(d.sum() - m.n_filled) < 1e-12
+    rho = sec.one_body(c)
+    ref = np.array([[np.vdot(psi, hop(psi, i, j)) for j in range(N)] for i in range(N)])
+    assert np.abs(rho - ref).max() < 1e-12
+    assert np.abs(rho - rho.conj().T).max() < 1e-12 and abs(np.trace(rho) - m.n_filled) < 1e-12
