@@ -158,6 +158,13 @@ int sed_sector_density_correlation(sed_sector* sec, int which, double* nn, doubl
  * of the last sed_solve (complex interleaved, 0-based sites), the quantity of
  * off_diagonal_long_range_order (observables/off_diagonal_long_range_order.jl:55-126). */
 int sed_sector_one_body(sed_sector* sec, int which, double* rho);
+/* Matrix mode: the sector block of the Hamiltonian set by sed_sector_set_terms as the
+ * reference's SparseMatrixCSC{ComplexF64,Int} (build_ed_Hamiltonian_symmetry_block,
+ * src/symmetry_resolved_ed.jl:903-963): 1-based colptr (dim+1) and rowval (nnz), nzval
+ * (nnz ComplexF64), duplicates summed, explicit zeros dropped.  Call once with colptr,
+ * rowval and nzval NULL to get *nnz (the matrix is built and kept), then again with
+ * arrays of that size to receive it (the kept copy is released). */
+int sed_sector_matrix_csc(sed_sector* sec, int64_t* nnz, int64_t* colptr, int64_t* rowval, double* nzval);
 int sed_sector_destroy(sed_sector* sec);
 
 /* Continuous-batching service of many sector solves (no reference counterpart; it

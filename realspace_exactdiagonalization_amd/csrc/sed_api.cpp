@@ -25,6 +25,8 @@ template <typename R>
 size_t matvec_lds_bytes(const sed_sector* sec);
 int launch_density(sed_sector* sec, const double2* vec, double* density);
 int launch_pair_density(sed_sector* sec, const double2* vec, std::vector<double>& M);
+int build_sector_csc(sed_sector* sec, std::vector<int64_t>& colptr, std::vector<int64_t>& rowval,
+                     std::vector<double>& nzval);
 int launch_cdot(const double2* x, const double2* y, int64_t n, double2* part, std::vector<double2>& host,
                 hipStream_t s, cd* out);
 int launch_convert(const double* a, float* b, int64_t n, hipStream_t s);
@@ -465,6 +467,10 @@ int sed_sector_set_terms(sed_sector* sec, int n_bil, const int32_t* i_from, cons
         return fail(SED_EUNSUPPORTED, "term lists need " + std::to_string(need) + " bytes of LDS per workgroup");
     sec->has_terms = true;
     sec->hops_gated = sec->hops_in_sector = -1;
+    sec->csc_ready = false;   // a kept matrix belongs to the previous terms
+    std::vector<int64_t>().swap(sec->csc_colptr);
+    std::vector<int64_t>().swap(sec->csc_rowval);
+    std::vector<double>().swap(sec->csc_nzval);
     // process-wide unique: a captured Lanczos step keyed on (sector address, version)
     // can never match a later sector that reuses a destroyed sector's address
     sec->version = ++g_term_version;
@@ -780,6 +786,28 @@ int sed_sector_one_body(sed_sector* sec, int which, double* rho) {
     part.release();
     x.release();
     return rc;
+}
+
+int sed_sector_matrix_csc(sed_sector* sec, int64_t* nnz, int64_t* colptr, int64_t* rowval, double* nzval) {
+    if (!sec || !nnz) return fail(SED_EINVAL, "NULL argument");
+    if (!sec->has_terms) return fail(SED_ESTATE, "matrix before sed_sector_set_terms");
+    const bool query = !colptr && !rowval && !nzval;
+    if (!query && (!colptr || !rowval || !nzval)) return fail(SED_EINVAL, "colptr, rowval and nzval: all or none");
+    SED_HIP(hipSetDevice(sec->cat->ctx->device));
+    if (!sec->csc_ready) {
+        SED_TRY(build_sector_csc(sec, sec->csc_colptr, sec->csc_rowval, sec->csc_nzval));
+        sec->csc_ready = true;
+    }
+    *nnz = (int64_t)sec->csc_rowval.size();
+    if (query) return SED_OK;
+    for (size_t i = 0; i < sec->csc_colptr.size(); ++i) colptr[i] = sec->csc_colptr[i] + 1;
+    for (size_t i = 0; i < sec->csc_rowval.size(); ++i) rowval[i] = sec->csc_rowval[i] + 1;
+    std::memcpy(nzval, sec->csc_nzval.data(), sec->csc_nzval.size() * sizeof(double));
+    std::vector<int64_t>().swap(sec->csc_colptr);
+    std::vector<int64_t>().swap(sec->csc_rowval);
+    std::vector<double>().swap(sec->csc_nzval);
+    sec->csc_ready = false;
+    return SED_OK;
 }
 
 int sed_sector_destroy(sed_sector* sec) {

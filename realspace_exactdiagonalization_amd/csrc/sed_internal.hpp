@@ -128,6 +128,10 @@ struct sed_sector {
     std::vector<double> last_evals;
     std::vector<sed::cd> last_evecs;  // dim x nev, reference basis
     int last_nev = 0;
+    // matrix mode: merged CSC kept between the size query and the copy (0-based)
+    std::vector<int64_t> csc_colptr, csc_rowval;
+    std::vector<double> csc_nzval;
+    bool csc_ready = false;
 };
 
 namespace sed {

@@ -32,7 +32,7 @@ __all__ = [
     "build_symmetry_orbit_catalog", "build_symmetry_sector_basis", "hamiltonian_linear_operator",
     "diagonalize_block_matrixfree", "build_ed_data", "ed_scan_at_irrep_matrixfree", "ed_scan_flux",
     "SectorServer", "PREC_FP64", "PREC_FP32", "PREC_FP32_REFINE", "sector_density",
-    "density_correlation_matrix", "off_diagonal_long_range_order",
+    "density_correlation_matrix", "off_diagonal_long_range_order", "build_ed_Hamiltonian_symmetry_block",
 ]
 
 _default_ctx = None
@@ -268,6 +268,23 @@ def sector_density(H_op: MatrixFreeHamiltonian, which: int = 0) -> np.ndarray:
     d = np.zeros(H_op.basis.catalog.n_site, dtype=np.float64)
     check(lib().sed_sector_density(H_op.basis._h, which, d))
     return d
+
+
+def build_ed_Hamiltonian_symmetry_block(basis: SymmetrySectorBasis, bilinear_terms, density_terms, cmap=None):
+    """Matrix mode (``src/symmetry_resolved_ed.jl:903-963``): the sector block as a
+    ``scipy.sparse.csc_matrix`` (0-based; duplicates summed, explicit zeros dropped),
+    built on the GPU from the device tables.  ``cmap`` is accepted and unused."""
+    import scipy.sparse as sp
+    basis.set_terms(bilinear_terms, density_terms)
+    nnz = C.c_int64()
+    check(lib().sed_sector_matrix_csc(basis._h, C.byref(nnz), None, None, None))
+    n = basis.dim
+    colptr = np.zeros(n + 1, dtype=np.int64)
+    rowval = np.zeros(nnz.value, dtype=np.int64)
+    nzval = np.zeros(nnz.value, dtype=np.complex128)
+    check(lib().sed_sector_matrix_csc(basis._h, C.byref(nnz), colptr.ctypes.data, rowval.ctypes.data,
+                                      nzval.ctypes.data))
+    return sp.csc_matrix((nzval, rowval - 1, colptr - 1), shape=(n, n))
 
 
 def density_correlation_matrix(H_op: MatrixFreeHamiltonian, which: int = 0, subtract_disconnected: bool = True):

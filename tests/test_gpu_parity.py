@@ -418,3 +418,43 @@ def test_density_correlation_and_one_body_match_oracle(ed, ctx, oracle_mod, name
     assert abs(np.trace(rho) - m.n_filled) < 1e-10
     gb.close()
     gc.close()
+
+
+@pytest.mark.parametrize("name", ["hald2x3", "hub2x3", "checker3x2_f_flux", "checker3x2_b_flux", "hald3x2_flux"])
+def test_matrix_mode_block_matches_oracle(ed, ctx, oracle_mod, golden, name):
+    """Matrix mode (build_ed_Hamiltonian_symmetry_block) from the GPU tables against the
+    oracle's sector matrix (projection table + diagonal, duplicates summed) for two
+    momenta, to 1e-13; Hermitian; equal to the matrix-free operator on a random vector;
+    for Haldane 2x3 at k=(0,0) the stored nnz of the reference's block (756)."""
+    m, G = make(name)
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, m.statistics == L.FERMIONIC)
+    irs = L.translation_irreps(G, m.lattice)
+    rng = np.random.default_rng(3)
+    for ir in (irs[0], irs[len(irs) // 2]):
+        gb = ed.build_symmetry_sector_basis(gc, ir)
+        print(f"[matrix mode] {name} k={ir.label} dim={gb.dim}", flush=True)
+        A = ed.build_ed_Hamiltonian_symmetry_block(gb, m.bilinear_terms, m.density_terms)
+        os_ = oracle_mod.Sector(oc, ir.values)
+        os_.set_terms(*L.term_arrays(m))
+        B = os_.sparse().tocsc()
+        B.eliminate_zeros()
+        scale = max(abs(B).max(), 1.0)
+        assert abs(A - B).max() <= 1e-13 * scale
+        # entries that cancel to zero in one summation order can be ~1e-17 in another
+        # (twisted phases): equal patterns above the tolerance, exact nnz without twist
+        big = lambda M: int((abs(M.tocoo().data) > 1e-13 * scale).sum())
+        assert big(A) == big(B)
+        if all(abs(p - 1) == 0 for p in G.phases.reshape(-1)):
+            assert A.nnz == B.nnz
+        assert abs(A - A.conj().T).max() <= 1e-13 * scale
+        H = ed.MatrixFreeHamiltonian.__new__(ed.MatrixFreeHamiltonian)
+        H.basis, H.n, H.precision = gb, gb.dim, ed.PREC_FP64
+        x = rng.standard_normal(gb.dim) + 1j * rng.standard_normal(gb.dim)
+        assert np.abs(A @ x - H(x)).max() <= 1e-12 * np.abs(A @ x).max()
+        if name == "hald2x3" and ir is irs[0]:
+            assert A.shape[0] == golden["design_notebook"]["haldane_2x3"]["sector_dim"]
+            assert A.nnz == golden["design_notebook"]["haldane_2x3"]["nnz_H"] == 756
+        gb.close()
+    gc.close()
