@@ -165,6 +165,21 @@ int sed_sector_one_body(sed_sector* sec, int which, double* rho);
  * rowval and nzval NULL to get *nnz (the matrix is built and kept), then again with
  * arrays of that size to receive it (the kept copy is released). */
 int sed_sector_matrix_csc(sed_sector* sec, int64_t* nnz, int64_t* colptr, int64_t* rowval, double* nzval);
+/* Fock amplitudes of a sector state c (dim ComplexF64, reference basis), as
+ * _expand_sector_state_to_fock_amplitudes (observables/entanglement_spectrum.jl:46-79):
+ * representatives with |c| <= atol skipped, amplitudes with |a| <= atol dropped, the
+ * rest normalised.  Writes the n_out nonzero (mask, amplitude) pairs in ascending mask
+ * order when capacity >= n_out; capacity 0 only reports n_out.  SED_EINVAL when
+ * 0 < capacity < n_out or the expanded norm is <= atol (the reference errors). */
+int sed_sector_fock_expansion(sed_sector* sec, const double* c, double atol, int64_t capacity, int64_t* n_out,
+                              uint64_t* masks, double* amps);
+/* Projected position operator <chi_to| U_d |chi_from> between two sectors of one
+ * catalog (_position_operator_matrix, observables/charge_pump.jl:47-91) with site
+ * phases phi (n_site ComplexF64): column col of the from-sector has its one possible
+ * entry in rows[col] (1-based row of the to-sector, 0 when the orbit is not there)
+ * with value vals[col] (ComplexF64). */
+int sed_sector_position_operator(sed_sector* to, sed_sector* from, const double* site_phases, int64_t* rows,
+                                 double* vals);
 int sed_sector_destroy(sed_sector* sec);
 
 /* Continuous-batching service of many sector solves (no reference counterpart; it

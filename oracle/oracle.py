@@ -89,6 +89,9 @@ def lib():
                                            _f64p, _f64p, _f64p, C.c_int, C.c_int64, C.c_int64]
         L.oc_density_correlation.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
         L.oc_one_body.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.oc_fock_expand.restype = C.c_int64
+        L.oc_fock_expand.argtypes = [C.c_void_p, _f64p, C.c_double, _f64p]
+        L.oc_position_operator.argtypes = [C.c_void_p, C.c_void_p, _f64p, _i64p, _f64p]
         _lib = L
     return _lib
 
@@ -299,6 +302,24 @@ class Sector:
         d = np.zeros(n)
         lib().oc_density_correlation(self._h, c.view(np.float64), C, d)
         return C.reshape(n, n), d
+
+    def fock_expand(self, c, atol: float = 1e-12):
+        """Rank-indexed (= ascending-mask) normalised Fock amplitudes of the sector state c
+        (_expand_sector_state_to_fock_amplitudes, entanglement_spectrum.jl:46-79); None when
+        the expanded norm is <= atol."""
+        c = np.ascontiguousarray(c, dtype=np.complex128)
+        out = np.zeros(self.catalog.n_total, dtype=np.complex128)
+        cnt = lib().oc_fock_expand(self._h, c.view(np.float64), atol, out.view(np.float64))
+        return None if cnt < 0 else out
+
+    def position_operator(self, sec_from, phi):
+        """(rows 1-based / 0, values) per column of sec_from for <chi_self| U_d |chi_from>
+        (_position_operator_matrix, charge_pump.jl:47-91)."""
+        phi = np.ascontiguousarray(phi, dtype=np.complex128)
+        rows = np.zeros(sec_from.dim, dtype=np.int64)
+        vals = np.zeros(sec_from.dim, dtype=np.complex128)
+        lib().oc_position_operator(self._h, sec_from._h, phi.view(np.float64), rows, vals.view(np.float64))
+        return rows, vals
 
     def one_body(self, c):
         """rho[i, j] = <psi| a_i^dag a_j |psi> of the sector state c

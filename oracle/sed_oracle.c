@@ -707,3 +707,66 @@ void oc_one_body(const oc_sector* s, const double* c_, double* rho_) {
         }
     }
 }
+
+/* _expand_sector_state_to_fock_amplitudes (observables/entanglement_spectrum.jl:46-79)
+ * into a rank-indexed dense array out[C(N,k)] (complex): representatives with
+ * |c| <= atol skipped, each image adds c/sqrt(|G| stab) conj(chi(g)) alpha_g(rep);
+ * returns the number of amplitudes with |a| > atol and normalises those (the others
+ * are set to zero), or -1 when the norm is <= atol (the reference errors). */
+int64_t oc_fock_expand(const oc_sector* s, const double* c_, double atol, double* out_) {
+    const cplx* c = (const cplx*)c_;
+    cplx* out = (cplx*)out_;
+    const oc_catalog* cat = s->cat;
+    const int nG = cat->G.n_group;
+    for (int64_t k = 0; k < cat->n_total; ++k) out[k] = 0.0;
+    for (int64_t col = 0; col < s->dim; ++col) {
+        const cplx coeff = c[col];
+        if (cabs(coeff) <= atol) continue;
+        const cplx prefactor = coeff / sqrt((double)nG * (double)s->stab[col]);
+        for (int g = 0; g < nG; ++g) {
+            cplx phase;
+            const mask_t shifted = apply_op(s->reps[col], &cat->G, g, &phase);
+            const cplx amp = (prefactor * conj(s->chi[g])) * phase;
+            out[oc_rank_combination(shifted)] += amp;
+        }
+    }
+    double norm2 = 0.0;
+    int64_t cnt = 0;
+    for (int64_t k = 0; k < cat->n_total; ++k) {
+        if (cabs(out[k]) <= atol) { out[k] = 0.0; continue; }
+        norm2 += creal(out[k]) * creal(out[k]) + cimag(out[k]) * cimag(out[k]);
+        ++cnt;
+    }
+    if (!(norm2 > atol)) return -1;
+    const double invnorm = 1.0 / sqrt(norm2);
+    for (int64_t k = 0; k < cat->n_total; ++k) out[k] *= invnorm;
+    return cnt;
+}
+
+/* _position_operator_matrix (observables/charge_pump.jl:47-91): per from-column, the
+ * to-sector row of the same representative (1-based, 0 if absent) and
+ * (1/|G|) sum_g chi_to(g) conj(chi_from(g)) z(g(rep)), z(mask) = prod phi_i. */
+void oc_position_operator(const oc_sector* to, const oc_sector* from, const double* phi_, int64_t* rows,
+                          double* vals_) {
+    const cplx* phi = (const cplx*)phi_;
+    cplx* vals = (cplx*)vals_;
+    const oc_catalog* cat = from->cat;
+    const int nG = cat->G.n_group;
+    const double inv_nG = 1.0 / nG;
+    for (int64_t col = 0; col < from->dim; ++col) {
+        const mask_t repr = from->reps[col];
+        const int64_t row = basis_index(to, repr);
+        rows[col] = row;
+        vals[col] = 0.0;
+        if (row == 0) continue;
+        cplx elem = 0.0;
+        for (int g = 0; g < nG; ++g) {
+            cplx a;
+            const mask_t shifted = apply_op(repr, &cat->G, g, &a);
+            cplx z = 1.0;
+            for (mask_t t = shifted; t; t &= t - 1) z *= phi[__builtin_ctzll(t)];
+            elem += (to->chi[g] * conj(from->chi[g])) * z;
+        }
+        vals[col] = elem * inv_nG;
+    }
+}

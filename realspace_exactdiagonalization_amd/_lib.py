@@ -86,6 +86,8 @@ SIGNATURES = {
     "sed_sector_density_correlation": (C.c_int, [_vp, C.c_int, _f64p, _f64p]),
     "sed_sector_one_body": (C.c_int, [_vp, C.c_int, _f64p]),
     "sed_sector_matrix_csc": (C.c_int, [_vp, C.POINTER(C.c_int64), _vp, _vp, _vp]),
+    "sed_sector_fock_expansion": (C.c_int, [_vp, _vp, C.c_double, C.c_int64, C.POINTER(C.c_int64), _vp, _vp]),
+    "sed_sector_position_operator": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "sed_sector_destroy": (C.c_int, [_vp]),
     "sed_server_create": (C.c_int, [_vp, C.c_int64, C.c_int, _pp]),
     "sed_server_submit": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int,

@@ -27,6 +27,9 @@ int launch_density(sed_sector* sec, const double2* vec, double* density);
 int launch_pair_density(sed_sector* sec, const double2* vec, std::vector<double>& M);
 int build_sector_csc(sed_sector* sec, std::vector<int64_t>& colptr, std::vector<int64_t>& rowval,
                      std::vector<double>& nzval);
+int launch_fock_expand(sed_sector* sec, const double2* c, double atol, double2* out, hipStream_t s);
+int launch_position(sed_sector* to, sed_sector* from, const double2* phi, int64_t* rows, double2* vals,
+                    hipStream_t s);
 int launch_cdot(const double2* x, const double2* y, int64_t n, double2* part, std::vector<double2>& host,
                 hipStream_t s, cd* out);
 int launch_convert(const double* a, float* b, int64_t n, hipStream_t s);
@@ -807,6 +810,89 @@ int sed_sector_matrix_csc(sed_sector* sec, int64_t* nnz, int64_t* colptr, int64_
     std::vector<int64_t>().swap(sec->csc_rowval);
     std::vector<double>().swap(sec->csc_nzval);
     sec->csc_ready = false;
+    return SED_OK;
+}
+
+// mask of colex rank r (k-subsets of {0..n-1}), from the catalog's binomial table
+static uint64_t unrank_mask(const sed_catalog* cat, uint64_t r) {
+    uint64_t m = 0;
+    int b = cat->n_site - 1;
+    for (int j = cat->n_filled; j >= 1; --j) {
+        while (cat->binom[(size_t)b * cat->bstride + j] > r) --b;
+        m |= (uint64_t)1 << b;
+        r -= cat->binom[(size_t)b * cat->bstride + j];
+        --b;
+    }
+    return m;
+}
+
+int sed_sector_fock_expansion(sed_sector* sec, const double* c, double atol, int64_t capacity, int64_t* n_out,
+                              uint64_t* masks, double* amps) {
+    if (!sec || !c || !n_out) return fail(SED_EINVAL, "NULL argument");
+    if (capacity < 0 || (capacity > 0 && (!masks || !amps))) return fail(SED_EINVAL, "bad output arrays");
+    const sed_catalog* cat = sec->cat;
+    SED_HIP(hipSetDevice(cat->ctx->device));
+    hipStream_t s = cat->ctx->stream;
+    const int64_t n = sec->dim, nt = cat->n_total;
+    *n_out = 0;
+    if (n == 0) return fail(SED_EINVAL, "Expanded state has near-zero norm.");
+    DevBuf<double2> dc, out;
+    SED_TRY(dc.alloc((size_t)n));
+    SED_TRY(out.alloc((size_t)nt));
+    SED_HIP(hipMemcpyAsync(dc.p, c, (size_t)n * 16, hipMemcpyHostToDevice, s));
+    SED_TRY(launch_fock_expand(sec, dc.p, atol, out.p, s));
+    std::vector<double2> h((size_t)nt);
+    SED_HIP(hipMemcpy(h.data(), out.p, (size_t)nt * 16, hipMemcpyDeviceToHost));
+    dc.release();
+    out.release();
+    double norm2 = 0.0;
+    int64_t cnt = 0;
+    for (int64_t k = 0; k < nt; ++k) {
+        const double a = std::hypot(h[k].x, h[k].y);
+        if (a > atol) {
+            norm2 += h[k].x * h[k].x + h[k].y * h[k].y;
+            ++cnt;
+        }
+    }
+    if (!(norm2 > atol)) return fail(SED_EINVAL, "Expanded state has near-zero norm.");
+    *n_out = cnt;
+    if (capacity == 0) return SED_OK;
+    if (capacity < cnt) return fail(SED_EINVAL, "capacity " + std::to_string(capacity) + " < " + std::to_string(cnt));
+    const double inv = 1.0 / std::sqrt(norm2);
+    int64_t q = 0;
+    for (int64_t k = 0; k < nt; ++k) {
+        if (!(std::hypot(h[k].x, h[k].y) > atol)) continue;
+        masks[q] = unrank_mask(cat, (uint64_t)k);   // colex rank order = ascending masks
+        amps[2 * q] = h[k].x * inv;
+        amps[2 * q + 1] = h[k].y * inv;
+        ++q;
+    }
+    return SED_OK;
+}
+
+int sed_sector_position_operator(sed_sector* to, sed_sector* from, const double* site_phases, int64_t* rows,
+                                 double* vals) {
+    if (!to || !from || !site_phases || !rows || !vals) return fail(SED_EINVAL, "NULL argument");
+    if (to->cat != from->cat) return fail(SED_EINVAL, "sectors of different catalogs (symmetry groups)");
+    const sed_catalog* cat = from->cat;
+    SED_HIP(hipSetDevice(cat->ctx->device));
+    hipStream_t s = cat->ctx->stream;
+    const int64_t n = from->dim;
+    if (n == 0) return SED_OK;
+    DevBuf<double2> phi, dv;
+    DevBuf<int64_t> dr;
+    SED_TRY(phi.alloc((size_t)cat->n_site));
+    SED_TRY(dv.alloc((size_t)n));
+    SED_TRY(dr.alloc((size_t)n));
+    SED_HIP(hipMemcpyAsync(phi.p, site_phases, (size_t)cat->n_site * 16, hipMemcpyHostToDevice, s));
+    SED_TRY(launch_position(to, from, phi.p, dr.p, dv.p, s));
+    SED_HIP(hipMemcpyAsync(rows, dr.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    SED_HIP(hipMemcpyAsync(vals, dv.p, (size_t)n * 16, hipMemcpyDeviceToHost, s));
+    SED_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < n; ++i) rows[i] += 1;   // 1-based, 0 = no entry
+    phi.release();
+    dv.release();
+    dr.release();
     return SED_OK;
 }
 

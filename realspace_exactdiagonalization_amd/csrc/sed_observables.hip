@@ -199,6 +199,133 @@ int launch_cdot(const double2* x, const double2* y, int64_t n, double2* part, st
     return SED_OK;
 }
 
+// Sector state -> Fock amplitudes (observables/entanglement_spectrum.jl:46-79): every
+// representative r with |c_r| > atol adds, for each group element g,
+//   c_r / sqrt(|G| stab_r) * conj(chi(g)) * alpha_g(rep_r)
+// to the amplitude of the image g(rep_r), alpha = product of the site phases of the
+// occupied sites times the reordering sign.  One thread owns one representative and
+// its whole orbit (orbits are disjoint), so it adds its images in order into the
+// rank-indexed output without races.
+template <typename M, bool FERMI>
+__global__ void __launch_bounds__(kObsBlock) k_fock_expand(const uint64_t* __restrict__ rep,
+                                                           const uint8_t* __restrict__ stab,
+                                                           const double2* __restrict__ c, int64_t dim, double atol,
+                                                           int n, int nG, const uint8_t* __restrict__ perm,
+                                                           const double2* __restrict__ phases,
+                                                           const double2* __restrict__ chi,
+                                                           const uint32_t* __restrict__ bin, int bs,
+                                                           double2* __restrict__ out) {
+    __shared__ uint8_t lp[64 * 64];
+    __shared__ uint32_t lb[65 * 66];
+    for (int i = threadIdx.x; i < n * nG; i += blockDim.x) lp[i] = perm[i];
+    for (int i = threadIdx.x; i < (n + 1) * bs; i += blockDim.x) lb[i] = bin[i];
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= dim) return;
+    const double2 cr = c[r];
+    if (!(sqrt(cr.x * cr.x + cr.y * cr.y) > atol)) return;
+    const double pre = 1.0 / sqrt((double)nG * (double)stab[r]);
+    const M m = (M)rep[r];
+    for (int g = 0; g < nG; ++g) {
+        int par = 0;
+        const M img = apply_perm<M, FERMI>(m, lp + g * n, par);
+        double2 ph = make_double2(1.0, 0.0);
+        for (M t = m; t; t &= t - 1) ph = cmul(ph, phases[g * n + ctz_m<M>(t)]);
+        const double2 xc = make_double2(chi[g].x, -chi[g].y);
+        double2 a = cmul(cmul(make_double2(cr.x * pre, cr.y * pre), xc), ph);
+        if (FERMI && par) a = make_double2(-a.x, -a.y);
+        const uint32_t k = colex_rank<M>(img, lb, bs);
+        out[k].x += a.x;
+        out[k].y += a.y;
+    }
+}
+
+int launch_fock_expand(sed_sector* sec, const double2* c, double atol, double2* out, hipStream_t s) {
+    const sed_catalog* cat = sec->cat;
+    const int n = cat->n_site, nG = cat->n_group;
+    std::vector<double2> ph((size_t)nG * n);
+    for (size_t i = 0; i < ph.size(); ++i) ph[i] = make_double2(cat->phases[i].real(), cat->phases[i].imag());
+    DevBuf<double2> dph;
+    SED_TRY(dph.alloc(ph.size()));
+    SED_HIP(hipMemcpyAsync(dph.p, ph.data(), ph.size() * 16, hipMemcpyHostToDevice, s));
+    SED_HIP(hipMemsetAsync(out, 0, (size_t)cat->n_total * 16, s));
+    const unsigned nb = (unsigned)std::max<int64_t>(1, (sec->dim + kObsBlock - 1) / kObsBlock);
+    const auto* chi = reinterpret_cast<const double2*>(sec->d_chi64.p);
+    const bool F = cat->stats == SED_FERMIONIC;
+    if (n <= 32) {
+        if (F) hipLaunchKernelGGL((k_fock_expand<uint32_t, true>), dim3(nb), dim3(kObsBlock), 0, s, sec->d_rep.p, sec->d_stab.p, c, sec->dim, atol, n, nG, cat->d_perm.p, dph.p, chi, cat->d_binom.p, cat->bstride, out);
+        else hipLaunchKernelGGL((k_fock_expand<uint32_t, false>), dim3(nb), dim3(kObsBlock), 0, s, sec->d_rep.p, sec->d_stab.p, c, sec->dim, atol, n, nG, cat->d_perm.p, dph.p, chi, cat->d_binom.p, cat->bstride, out);
+    } else {
+        if (F) hipLaunchKernelGGL((k_fock_expand<uint64_t, true>), dim3(nb), dim3(kObsBlock), 0, s, sec->d_rep.p, sec->d_stab.p, c, sec->dim, atol, n, nG, cat->d_perm.p, dph.p, chi, cat->d_binom.p, cat->bstride, out);
+        else hipLaunchKernelGGL((k_fock_expand<uint64_t, false>), dim3(nb), dim3(kObsBlock), 0, s, sec->d_rep.p, sec->d_stab.p, c, sec->dim, atol, n, nG, cat->d_perm.p, dph.p, chi, cat->d_binom.p, cat->bstride, out);
+    }
+    SED_HIP(hipGetLastError());
+    SED_HIP(hipStreamSynchronize(s));
+    dph.release();
+    return SED_OK;
+}
+
+// Projected position operator <chi_to| U_d |chi_from> (observables/charge_pump.jl:47-91):
+// column col of the from-sector has at most one entry, in the to-sector row of the same
+// representative, (1/|G|) sum_g chi_to(g) conj(chi_from(g)) z(g(rep)) with
+// z(mask) = prod_{i in mask} phi_i; row = -1 when the orbit is not in the to-sector.
+template <typename M>
+__global__ void __launch_bounds__(kObsBlock) k_position(const uint64_t* __restrict__ rep, int64_t dim, int n, int nG,
+                                                        const uint8_t* __restrict__ perm,
+                                                        const double2* __restrict__ phi,
+                                                        const double2* __restrict__ chi_to,
+                                                        const double2* __restrict__ chi_from,
+                                                        const uint32_t* __restrict__ table_to, int shift_to,
+                                                        const uint32_t* __restrict__ bin, int bs,
+                                                        int64_t* __restrict__ rows, double2* __restrict__ vals) {
+    __shared__ uint8_t lp[64 * 64];
+    __shared__ uint32_t lb[65 * 66];
+    for (int i = threadIdx.x; i < n * nG; i += blockDim.x) lp[i] = perm[i];
+    for (int i = threadIdx.x; i < (n + 1) * bs; i += blockDim.x) lb[i] = bin[i];
+    __syncthreads();
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= dim) return;
+    const M m = (M)rep[col];
+    const uint32_t e = table_to[colex_rank<M>(m, lb, bs)];
+    if (e == kNoRow) {
+        rows[col] = -1;
+        vals[col] = make_double2(0.0, 0.0);
+        return;
+    }
+    double2 el = make_double2(0.0, 0.0);
+    for (int g = 0; g < nG; ++g) {
+        int par = 0;
+        const M img = apply_perm<M, false>(m, lp + g * n, par);
+        double2 z = make_double2(1.0, 0.0);
+        for (M t = img; t; t &= t - 1) z = cmul(z, phi[ctz_m<M>(t)]);
+        const double2 w = cmul(chi_to[g], make_double2(chi_from[g].x, -chi_from[g].y));
+        const double2 v = cmul(w, z);
+        el.x += v.x;
+        el.y += v.y;
+    }
+    rows[col] = (int64_t)(e >> shift_to);
+    vals[col] = make_double2(el.x / nG, el.y / nG);
+}
+
+int launch_position(sed_sector* to, sed_sector* from, const double2* phi, int64_t* rows, double2* vals,
+                    hipStream_t s) {
+    const sed_catalog* cat = from->cat;
+    const int n = cat->n_site, nG = cat->n_group;
+    const unsigned nb = (unsigned)std::max<int64_t>(1, (from->dim + kObsBlock - 1) / kObsBlock);
+    const auto* cto = reinterpret_cast<const double2*>(to->d_chi64.p);
+    const auto* cfr = reinterpret_cast<const double2*>(from->d_chi64.p);
+    if (n <= 32)
+        hipLaunchKernelGGL((k_position<uint32_t>), dim3(nb), dim3(kObsBlock), 0, s, from->d_rep.p, from->dim, n, nG,
+                           cat->d_perm.p, phi, cto, cfr, to->d_table.p, to->shift, cat->d_binom.p, cat->bstride, rows,
+                           vals);
+    else
+        hipLaunchKernelGGL((k_position<uint64_t>), dim3(nb), dim3(kObsBlock), 0, s, from->d_rep.p, from->dim, n, nG,
+                           cat->d_perm.p, phi, cto, cfr, to->d_table.p, to->shift, cat->d_binom.p, cat->bstride, rows,
+                           vals);
+    SED_HIP(hipGetLastError());
+    return SED_OK;
+}
+
 __global__ void __launch_bounds__(256) k_d2f(const double2* __restrict__ a, float2* __restrict__ b, int64_t n) {
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
         b[r] = make_float2((float)a[r].x, (float)a[r].y);

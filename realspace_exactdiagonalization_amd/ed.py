@@ -33,6 +33,7 @@ __all__ = [
     "diagonalize_block_matrixfree", "build_ed_data", "ed_scan_at_irrep_matrixfree", "ed_scan_flux",
     "SectorServer", "PREC_FP64", "PREC_FP32", "PREC_FP32_REFINE", "sector_density",
     "density_correlation_matrix", "off_diagonal_long_range_order", "build_ed_Hamiltonian_symmetry_block",
+    "expand_sector_state_to_fock_amplitudes", "fock_amplitudes", "position_operator_matrix",
 ]
 
 _default_ctx = None
@@ -285,6 +286,46 @@ def build_ed_Hamiltonian_symmetry_block(basis: SymmetrySectorBasis, bilinear_ter
     check(lib().sed_sector_matrix_csc(basis._h, C.byref(nnz), colptr.ctypes.data, rowval.ctypes.data,
                                       nzval.ctypes.data))
     return sp.csc_matrix((nzval, rowval - 1, colptr - 1), shape=(n, n))
+
+
+def expand_sector_state_to_fock_amplitudes(c, basis: SymmetrySectorBasis, atol: float = 1e-12) -> Dict[int, complex]:
+    """``_expand_sector_state_to_fock_amplitudes`` (observables/entanglement_spectrum.jl:46-79):
+    ``{mask: amplitude}`` of the sector state ``c`` in the fixed-N Fock basis, normalised,
+    amplitudes with ``|a| <= atol`` dropped.  Expanded on the GPU."""
+    masks, amps = fock_amplitudes(c, basis, atol)
+    return {int(m): complex(a) for m, a in zip(masks, amps)}
+
+
+def fock_amplitudes(c, basis: SymmetrySectorBasis, atol: float = 1e-12):
+    """Arrays ``(masks, amplitudes)`` (ascending masks) of the same expansion."""
+    c = np.ascontiguousarray(c, dtype=np.complex128)
+    if c.shape != (basis.dim,):
+        raise ValueError(f"expected a vector of length {basis.dim}")
+    n = C.c_int64()
+    check(lib().sed_sector_fock_expansion(basis._h, c.ctypes.data, atol, 0, C.byref(n), None, None))
+    masks = np.zeros(n.value, dtype=np.uint64)
+    amps = np.zeros(n.value, dtype=np.complex128)
+    check(lib().sed_sector_fock_expansion(basis._h, c.ctypes.data, atol, n.value, C.byref(n), masks.ctypes.data,
+                                          amps.ctypes.data))
+    return masks, amps
+
+
+def position_operator_matrix(basis_to: SymmetrySectorBasis, basis_from: SymmetrySectorBasis, site_phases):
+    """``_position_operator_matrix`` (observables/charge_pump.jl:47-91): the projected
+    position operator <chi_to| U_d |chi_from> as a ``scipy.sparse.csc_matrix`` (entries
+    with ``|elem| > 1e-13``), formed on the GPU."""
+    import scipy.sparse as sp
+    phi = np.ascontiguousarray(site_phases, dtype=np.complex128)
+    if phi.shape != (basis_from.catalog.n_site,):
+        raise ValueError("site_phases must have n_site entries")
+    nf = basis_from.dim
+    rows = np.zeros(nf, dtype=np.int64)
+    vals = np.zeros(nf, dtype=np.complex128)
+    check(lib().sed_sector_position_operator(basis_to._h, basis_from._h, phi.ctypes.data, rows.ctypes.data,
+                                             vals.ctypes.data))
+    keep = (rows > 0) & (np.abs(vals) > 1e-13)
+    cols = np.nonzero(keep)[0]
+    return sp.csc_matrix((vals[keep], (rows[keep] - 1, cols)), shape=(basis_to.dim, nf))
 
 
 def density_correlation_matrix(H_op: MatrixFreeHamiltonian, which: int = 0, subtract_disconnected: bool = True):

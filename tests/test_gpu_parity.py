@@ -458,3 +458,40 @@ def test_matrix_mode_block_matches_oracle(ed, ctx, oracle_mod, golden, name):
             assert A.nnz == golden["design_notebook"]["haldane_2x3"]["nnz_H"] == 756
         gb.close()
     gc.close()
+
+
+@pytest.mark.parametrize("name", ["checker3x2_f_flux", "checker3x2_b_flux", "hub2x3", "hald3x2_flux"])
+def test_fock_expansion_and_position_operator_match_oracle(ed, ctx, oracle_mod, name):
+    """Sector -> Fock expansion and the projected position operator on the GPU against
+    the oracle's restatements (pinned against the Fock-space embedding) to 1e-12:
+    same masks in ascending order, same amplitudes; same rows and values per column."""
+    m, G = make(name)
+    irs = L.translation_irreps(G, m.lattice)
+    print(f"[fock/position] {name}", flush=True)
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, m.statistics == L.FERMIONIC)
+    gb0, gb1 = ed.build_symmetry_sector_basis(gc, irs[1 % len(irs)]), ed.build_symmetry_sector_basis(gc, irs[0])
+    os0 = oracle_mod.Sector(oc, irs[1 % len(irs)].values)
+    os1 = oracle_mod.Sector(oc, irs[0].values)
+    rng = np.random.default_rng(5)
+    c = rng.standard_normal(gb0.dim) + 1j * rng.standard_normal(gb0.dim)
+    masks, amps = ed.fock_amplitudes(c, gb0)
+    ref = os0.fock_expand(c)
+    nz = np.nonzero(ref)[0]
+    assert len(masks) == len(nz) and np.all(np.diff(masks.astype(np.float64)) > 0)
+    assert all(oracle_mod.rank_combination(int(x)) == int(k) for x, k in zip(masks, nz))
+    assert np.abs(amps - ref[nz]).max() < 1e-12
+    d = ed.expand_sector_state_to_fock_amplitudes(c, gb0)
+    assert len(d) == len(nz) and abs(sum(abs(v) ** 2 for v in d.values()) - 1) < 1e-12
+    phi = np.exp(2j * np.pi * rng.random(m.n_site))
+    P = ed.position_operator_matrix(gb1, gb0, phi)
+    rows, vals = os1.position_operator(os0, phi)
+    dense = np.zeros((gb1.dim, gb0.dim), dtype=np.complex128)
+    for col, (r, v) in enumerate(zip(rows, vals)):
+        if r > 0 and abs(v) > 1e-13:
+            dense[r - 1, col] = v
+    assert np.abs(P.toarray() - dense).max() < 1e-12
+    gb0.close()
+    gb1.close()
+    gc.close()

@@ -357,3 +357,43 @@ def test_sector_observables_against_fock_space(oracle_mod, name):
     ref = np.array([[np.vdot(psi, hop(psi, i, j)) for j in range(N)] for i in range(N)])
     assert np.abs(rho - ref).max() < 1e-12
     assert np.abs(rho - rho.conj().T).max() < 1e-12 and abs(np.trace(rho) - m.n_filled) < 1e-12
+
+
+@pytest.mark.parametrize("name", list(OBS_CASES))
+def test_fock_expansion_and_position_operator_against_embedding(oracle_mod, name):
+    """The oracle's Fock expansion (entanglement_spectrum.jl:46-79) equals the verified
+    embedding B c (normalised), and its projected position operator
+    (charge_pump.jl:47-91) equals B_to^dag diag(prod phi_i^n_i) B_from between two
+    momentum sectors, for random site phases."""
+    f, th, k_idx = OBS_CASES[name]
+    m = L.twisted(f(), th)
+    G = L.model_group(m, th)
+    irs = L.translation_irreps(G, m.lattice)
+    cat = oracle_mod.Catalog(m.n_site, m.n_filled, G, m.statistics == L.FERMIONIC)
+    states, index, _ = _fock_ops(m)
+    rng = np.random.default_rng(11)
+    secs, Bs = [], []
+    for ir in (irs[k_idx], irs[0]):
+        sec = oracle_mod.Sector(cat, ir.values)
+        sec.set_terms(*L.term_arrays(m))
+        B = _embedding(oracle_mod, cat, sec, states, index, G, ir.values)
+        assert np.abs(full_fock_hamiltonian(m) @ B - B @ sec.dense()).max() < 1e-11
+        secs.append(sec)
+        Bs.append(B)
+    # Fock expansion of a random sector state (rank-indexed)
+    c = rng.standard_normal(secs[0].dim) + 1j * rng.standard_normal(secs[0].dim)
+    amp = secs[0].fock_expand(c)
+    ref = Bs[0] @ c
+    ref /= np.linalg.norm(ref)
+    ranks = [oracle_mod.rank_combination(sum(1 << b for b in s)) for s in states]
+    assert np.abs(amp[ranks] - ref).max() < 1e-12
+    # position operator between the two sectors
+    phi = np.exp(2j * np.pi * rng.random(m.n_site))
+    rows, vals = secs[1].position_operator(secs[0], phi)
+    z = np.array([np.prod([phi[b] for b in s]) for s in states])
+    P = Bs[1].conj().T @ (z[:, None] * Bs[0])
+    dense = np.zeros_like(P)
+    for col, (r, v) in enumerate(zip(rows, vals)):
+        if r > 0:
+            dense[r - 1, col] = v
+    assert np.abs(dense - P).max() < 1e-12
