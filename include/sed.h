@@ -92,6 +92,16 @@ int sed_catalog_copy(const sed_catalog* cat, int32_t* rep_rank, int32_t* rep_sym
 /* stabilizer_g_indices_list / stabilizer_phases_list (:309-310) as padded
  * n_orbits x n_group arrays (1-based indices, 0 = padding). */
 int sed_catalog_stabilizers(const sed_catalog* cat, int32_t* stab_gidx, double* stab_phase);
+/* update_orbit_stabilizer_phases!(catalog, new_group, particle_statistics) (:338-369):
+ * replace the group's site phases (n_group x n_site ComplexF64, same permutations, e.g.
+ * another flux twist) without re-enumerating orbits: the orbit tables depend on the
+ * permutations only, and with phases of gauge form f(i)/f(perm_g(i)) a stabilizer
+ * element's phase on its representative is its reordering sign, so sector membership
+ * does not change either.  Later dumps (sed_catalog_copy, sed_catalog_stabilizers) and
+ * sectors built afterwards use the new phases; sectors built before keep theirs.
+ * SED_EUNSUPPORTED (catalog unchanged) when the phases are not of gauge form or would
+ * change the inverse elements. */
+int sed_catalog_update_phases(sed_catalog* cat, const double* phases);
 int sed_catalog_destroy(sed_catalog* cat);
 
 /* build_symmetry_sector_basis(catalog, irrep; atol) (:544-556) with the irrep's

@@ -89,6 +89,8 @@ def lib():
                                            _f64p, _f64p, _f64p, C.c_int, C.c_int64, C.c_int64]
         L.oc_density_correlation.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
         L.oc_one_body.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.oc_catalog_update_phases.restype = C.c_int
+        L.oc_catalog_update_phases.argtypes = [C.c_void_p, _f64p]
         L.oc_fock_expand.restype = C.c_int64
         L.oc_fock_expand.argtypes = [C.c_void_p, _f64p, C.c_double, _f64p]
         L.oc_position_operator.argtypes = [C.c_void_p, C.c_void_p, _f64p, _i64p, _f64p]
@@ -171,6 +173,13 @@ class Catalog:
         gp = np.zeros((self.n_orbits, nG), dtype=np.complex128)
         lib().oc_catalog_orbits(self._h, reps, so, gi.reshape(-1), gp.view(np.float64).reshape(-1))
         return reps, so, gi, gp
+
+    def update_phases(self, new_group):
+        """``update_orbit_stabilizer_phases!`` (``src/symmetry_resolved_ed.jl:338-369``)."""
+        if lib().oc_catalog_update_phases(self._h, _c128(new_group.phases)) != 0:
+            raise RuntimeError("Symmetry operations are not closed under inversion.")
+        self.group = new_group
+        return self
 
     def canonical_representative(self, m: int):
         g = C.c_int32(0)

@@ -770,3 +770,35 @@ void oc_position_operator(const oc_sector* to, const oc_sector* from, const doub
         vals[col] = elem * inv_nG;
     }
 }
+
+/* update_orbit_stabilizer_phases! (src/symmetry_resolved_ed.jl:338-369): the group's
+ * site phases replaced in place (same permutations); stabilizer phases recomputed, and
+ * for every representative and group element (in order) whose image lies in the
+ * representative's orbit the rep_sym / rep_amp entries rewritten.  Returns 0, or -1
+ * when the new group is not closed under inversion. */
+int oc_catalog_update_phases(oc_catalog* c, const double* phases) {
+    oc_group* G = &c->G;
+    const int nG = G->n_group, n = G->n_site;
+    int32_t* inv = malloc((size_t)nG * sizeof(int32_t));
+    if (oc_inverse_indices(n, nG, G->perm, phases, inv) != 0) { free(inv); return -1; }
+    memcpy(G->phases, phases, (size_t)nG * n * sizeof(cplx));
+    memcpy(G->inv, inv, (size_t)nG * sizeof(int32_t));
+    free(inv);
+    for (int64_t o = 0; o < c->n_orbits; ++o)
+        for (int j = 0; j < nG && c->stab_gidx[o * nG + j] != 0; ++j) {
+            cplx a;
+            (void)apply_op(c->reps[o], G, c->stab_gidx[o * nG + j] - 1, &a);
+            c->stab_phase[o * nG + j] = a;
+        }
+    for (int64_t o = 0; o < c->n_orbits; ++o)
+        for (int g = 0; g < nG; ++g) {
+            cplx a;
+            const mask_t s = apply_op(c->reps[o], G, g, &a);
+            const int64_t idx = oc_rank_combination(s);
+            if (c->rep_rank[idx] == (int32_t)(o + 1)) {
+                c->rep_sym[idx] = G->inv[g];
+                c->rep_amp[idx] = conj(a);
+            }
+        }
+    return 0;
+}

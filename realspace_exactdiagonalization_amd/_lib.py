@@ -70,6 +70,7 @@ SIGNATURES = {
     "sed_catalog_copy": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sed_catalog_stabilizers": (C.c_int, [_vp, _vp, _vp]),
     "sed_catalog_destroy": (C.c_int, [_vp]),
+    "sed_catalog_update_phases": (C.c_int, [_vp, _f64p]),
     "sed_sector_build": (C.c_int, [_vp, _f64p, C.c_double, _pp, C.POINTER(C.c_int64)]),
     "sed_sector_basis": (C.c_int, [_vp, _vp, _vp]),
     "sed_sector_set_terms": (C.c_int, [_vp, C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _i32p, _f64p]),

@@ -124,41 +124,14 @@ int sed_destroy(sed_context* ctx) {
     return SED_OK;
 }
 
-int sed_catalog_build(sed_context* ctx, int n_site, int n_filled, int statistics, int n_group, int identity_idx,
-                      const int32_t* perm, const double* phases, sed_catalog** out) {
-    if (!ctx || !perm || !out) return fail(SED_EINVAL, "NULL argument");
-    *out = nullptr;
-    if (n_site < 1 || n_site > 64) return fail(SED_EUNSUPPORTED, "n_site must be in 1..64");
-    if (n_filled < 0 || n_filled > n_site) return fail(SED_EINVAL, "n_filled out of range");
-    if (statistics != SED_BOSONIC && statistics != SED_FERMIONIC) return fail(SED_EINVAL, "bad statistics");
-    if (n_group < 1 || n_group > 64) return fail(SED_EUNSUPPORTED, "group order must be in 1..64");
-    if (identity_idx < 1 || identity_idx > n_group) return fail(SED_EINVAL, "identity_idx out of range");
-    SED_HIP(hipSetDevice(ctx->device));
-    auto* cat = new sed_catalog();
-    std::unique_ptr<sed_catalog> guard(cat);
-    cat->ctx = ctx;
-    cat->n_site = n_site;
-    cat->n_filled = n_filled;
-    cat->stats = statistics;
-    cat->n_group = n_group;
-    cat->identity_idx = identity_idx;
-    cat->perm.assign(perm, perm + (size_t)n_group * n_site);
+// Site phases of the group (null = all 1): the inverse indices of
+// _inverse_operation_indices (:109-117) and the gauge form f of the phases.  The orbit
+// tables depend on the permutations only.
+static int set_group_phases(sed_catalog* cat, const double* phases) {
+    const int n_site = cat->n_site, n_group = cat->n_group;
     cat->phases.resize((size_t)n_group * n_site);
     for (size_t i = 0; i < cat->phases.size(); ++i)
         cat->phases[i] = phases ? cd(phases[2 * i], phases[2 * i + 1]) : cd(1.0, 0.0);
-    // permutations: 1-based, bijective
-    for (int g = 0; g < n_group; ++g) {
-        std::vector<char> seen(n_site, 0);
-        for (int i = 0; i < n_site; ++i) {
-            const int t = cat->perm[(size_t)g * n_site + i];
-            if (t < 1 || t > n_site || seen[t - 1])
-                return fail(SED_EINVAL, "perm of group element " + std::to_string(g + 1) + " is not a permutation");
-            seen[t - 1] = 1;
-        }
-    }
-    for (int i = 0; i < n_site; ++i)
-        if (cat->perm[(size_t)(identity_idx - 1) * n_site + i] != i + 1)
-            return fail(SED_EINVAL, "Identity must have trivial permutation");
     // _inverse_operation_indices (:109-117): first h with h o g = id and matching phases
     cat->inv.assign(n_group, -1);
     for (int g = 0; g < n_group; ++g) {
@@ -207,6 +180,42 @@ int sed_catalog_build(sed_context* ctx, int n_site, int n_filled, int statistics
             if (std::abs(cat->phases[(size_t)g * n_site + i] * cat->gauge_f[t] - cat->gauge_f[i]) > 1e-10)
                 return fail(SED_EUNSUPPORTED, "site phases are not of gauge form f(i)/f(perm(i))");
         }
+    return SED_OK;
+}
+
+int sed_catalog_build(sed_context* ctx, int n_site, int n_filled, int statistics, int n_group, int identity_idx,
+                      const int32_t* perm, const double* phases, sed_catalog** out) {
+    if (!ctx || !perm || !out) return fail(SED_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (n_site < 1 || n_site > 64) return fail(SED_EUNSUPPORTED, "n_site must be in 1..64");
+    if (n_filled < 0 || n_filled > n_site) return fail(SED_EINVAL, "n_filled out of range");
+    if (statistics != SED_BOSONIC && statistics != SED_FERMIONIC) return fail(SED_EINVAL, "bad statistics");
+    if (n_group < 1 || n_group > 64) return fail(SED_EUNSUPPORTED, "group order must be in 1..64");
+    if (identity_idx < 1 || identity_idx > n_group) return fail(SED_EINVAL, "identity_idx out of range");
+    SED_HIP(hipSetDevice(ctx->device));
+    auto* cat = new sed_catalog();
+    std::unique_ptr<sed_catalog> guard(cat);
+    cat->ctx = ctx;
+    cat->n_site = n_site;
+    cat->n_filled = n_filled;
+    cat->stats = statistics;
+    cat->n_group = n_group;
+    cat->identity_idx = identity_idx;
+    cat->perm.assign(perm, perm + (size_t)n_group * n_site);
+    // permutations: 1-based, bijective
+    for (int g = 0; g < n_group; ++g) {
+        std::vector<char> seen(n_site, 0);
+        for (int i = 0; i < n_site; ++i) {
+            const int t = cat->perm[(size_t)g * n_site + i];
+            if (t < 1 || t > n_site || seen[t - 1])
+                return fail(SED_EINVAL, "perm of group element " + std::to_string(g + 1) + " is not a permutation");
+            seen[t - 1] = 1;
+        }
+    }
+    for (int i = 0; i < n_site; ++i)
+        if (cat->perm[(size_t)(identity_idx - 1) * n_site + i] != i + 1)
+            return fail(SED_EINVAL, "Identity must have trivial permutation");
+    SED_TRY(set_group_phases(cat, phases));
     // binomials C(n, j), n <= n_site, j <= n_filled + 1 (saturating at 2^32-1)
     uint64_t tot = 1;
     {
@@ -330,6 +339,23 @@ int sed_catalog_stabilizers(const sed_catalog* cat, int32_t* stab_gidx, double* 
     return SED_OK;
 }
 
+int sed_catalog_update_phases(sed_catalog* cat, const double* phases) {
+    if (!cat) return fail(SED_EINVAL, "NULL catalog");
+    const std::vector<cd> old_phases = cat->phases, old_f = cat->gauge_f;
+    const std::vector<int32_t> old_inv = cat->inv;
+    const bool old_has = cat->has_phases;
+    int rc = set_group_phases(cat, phases);
+    if (rc == SED_OK && cat->inv != old_inv)
+        rc = fail(SED_EUNSUPPORTED, "the new phases change the inverse group elements (rebuild the catalog)");
+    if (rc != SED_OK) {
+        cat->phases = old_phases;
+        cat->gauge_f = old_f;
+        cat->inv = old_inv;
+        cat->has_phases = old_has;
+    }
+    return rc;
+}
+
 int sed_catalog_destroy(sed_catalog* cat) {
     if (!cat) return SED_OK;
     if (cat->n_sectors_alive > 0) {
@@ -357,6 +383,8 @@ int sed_sector_build(sed_catalog* cat, const double* chi, double atol, sed_secto
     SED_TRY(upload(sec->d_chi32, c32.data(), c32.size(), s));
     SED_TRY(launch_sector(sec, sec->d_chi64.p));
     sec->has_gauge = cat->has_phases;
+    sec->gauge_f = cat->gauge_f;
+    sec->phases = cat->phases;
     if (sec->has_gauge && sec->dim > 0) {
         std::vector<double> f(2 * cat->n_site);
         for (int i = 0; i < cat->n_site; ++i) {
@@ -406,7 +434,7 @@ int sed_sector_set_terms(sed_sector* sec, int n_bil, const int32_t* i_from, cons
         if (a < 1 || a > N || b < 1 || b > N) return fail(SED_EINVAL, "bilinear term site out of range");
         if (a == b) return fail(SED_EINVAL, "bilinear term with i_from == i_to changes the particle number");
         // gauge-transformed amplitude t' = t f(to)/f(from); the kernel uses conj(t')
-        const cd tp = cd(t[2 * k], t[2 * k + 1]) * cat->gauge_f[b - 1] / cat->gauge_f[a - 1];
+        const cd tp = cd(t[2 * k], t[2 * k + 1]) * sec->gauge_f[b - 1] / sec->gauge_f[a - 1];
         hops[{a - 1, b - 1}] += tp;
     }
     std::map<std::pair<int, int>, cd> dens;  // ordered by first site
@@ -729,6 +757,8 @@ int sed_sector_one_body(sed_sector* sec, int which, double* rho) {
     tmp.d_chi32 = sec->d_chi32;
     tmp.has_gauge = sec->has_gauge;
     tmp.d_gauge = sec->d_gauge;
+    tmp.gauge_f = sec->gauge_f;
+    tmp.phases = sec->phases;
     tmp.mv_shape[0] = sec->mv_shape[0] >= 0 ? sec->mv_shape[0] : 1;
     tmp.mv_chunk[0] = sec->mv_shape[0] >= 0 ? sec->mv_chunk[0] : 1;
     DevBuf<double2> y, xh, part;
@@ -743,7 +773,7 @@ int sed_sector_one_body(sed_sector* sec, int which, double* rho) {
             double ev[2] = {0.0, 0.0};
             for (int part_b = 0; part_b < 2 && rc == SED_OK; ++part_b) {
                 for (int g = 0; g < nG; ++g) {
-                    const cd tg = cat->phases[(size_t)g * n + i] * std::conj(cat->phases[(size_t)g * n + j]) /
+                    const cd tg = sec->phases[(size_t)g * n + i] * std::conj(sec->phases[(size_t)g * n + j]) /
                                   (double)nG;
                     const cd f = part_b ? cd(0.0, 1.0) * tg : tg;   // O_ij part; the O_ji part is its conjugate
                     from[2 * g] = cat->perm[(size_t)g * n + j];

@@ -113,6 +113,8 @@ struct sed_sector {
     std::vector<sed::cd> term_coef;   // conj(t'_k), gauge-transformed
     std::vector<sed::cd> dens_v;
     bool has_gauge = false;           // D = diag(F(rep)) != 1
+    std::vector<sed::cd> gauge_f;     // the catalog's f(i) and site phases when this sector
+    std::vector<sed::cd> phases;      // was built (a later flux refresh leaves them alone)
     sed::DevBuf<int32_t> d_term_start, d_dens_start;
     sed::DevBuf<uint8_t> d_term_to, d_dens_j;
     sed::DevBuf<double> d_term_coef64, d_dens_v64, d_chi64;  // interleaved complex

@@ -243,8 +243,8 @@ __global__ void __launch_bounds__(kObsBlock) k_fock_expand(const uint64_t* __res
 int launch_fock_expand(sed_sector* sec, const double2* c, double atol, double2* out, hipStream_t s) {
     const sed_catalog* cat = sec->cat;
     const int n = cat->n_site, nG = cat->n_group;
-    std::vector<double2> ph((size_t)nG * n);
-    for (size_t i = 0; i < ph.size(); ++i) ph[i] = make_double2(cat->phases[i].real(), cat->phases[i].imag());
+    std::vector<double2> ph((size_t)nG * n);   // the site phases the sector was built with
+    for (size_t i = 0; i < ph.size(); ++i) ph[i] = make_double2(sec->phases[i].real(), sec->phases[i].imag());
     DevBuf<double2> dph;
     SED_TRY(dph.alloc(ph.size()));
     SED_HIP(hipMemcpyAsync(dph.p, ph.data(), ph.size() * 16, hipMemcpyHostToDevice, s));

@@ -34,6 +34,7 @@ __all__ = [
     "SectorServer", "PREC_FP64", "PREC_FP32", "PREC_FP32_REFINE", "sector_density",
     "density_correlation_matrix", "off_diagonal_long_range_order", "build_ed_Hamiltonian_symmetry_block",
     "expand_sector_state_to_fock_amplitudes", "fock_amplitudes", "position_operator_matrix",
+    "update_orbit_stabilizer_phases",
 ]
 
 _default_ctx = None
@@ -115,6 +116,16 @@ class SymmetryOrbitCatalog:
         check(lib().sed_catalog_stabilizers(self._h, ptr(gi), ptr(gp)))
         return reps, so, gi, gp
 
+    def update_phases(self, new_group: SymmetryGroup):
+        """``update_orbit_stabilizer_phases!`` (``src/symmetry_resolved_ed.jl:338-369``): take the
+        site phases of ``new_group`` (same permutations) without re-enumerating orbits."""
+        if new_group.order != self.symmetry_group.order or not np.array_equal(
+                np.asarray(new_group.perm), np.asarray(self.symmetry_group.perm)):
+            raise ValueError("new_group must have the catalog's permutations")
+        check(lib().sed_catalog_update_phases(self._h, _c128(new_group.phases).reshape(-1)))
+        self.symmetry_group = new_group
+        return self
+
     def close(self):
         if getattr(self, "_h", None):
             lib().sed_catalog_destroy(self._h)
@@ -130,6 +141,12 @@ class SymmetryOrbitCatalog:
 def build_symmetry_orbit_catalog(*, n_site: int, n_filled: int, symmetry_group: SymmetryGroup,
                                  particle_statistics: int, ctx: Optional[Context] = None) -> SymmetryOrbitCatalog:
     return SymmetryOrbitCatalog(ctx or default_context(), n_site, n_filled, symmetry_group, particle_statistics)
+
+
+def update_orbit_stabilizer_phases(catalog: SymmetryOrbitCatalog, new_group: SymmetryGroup,
+                                   particle_statistics: Optional[int] = None) -> SymmetryOrbitCatalog:
+    """``update_orbit_stabilizer_phases!(catalog, new_group, particle_statistics)`` (``:338-369``)."""
+    return catalog.update_phases(new_group)
 
 
 class SymmetrySectorBasis:
@@ -433,22 +450,29 @@ class SectorServer:
 
 def ed_scan_flux(model: Model, thetas: Sequence[Tuple[float, float]], nev: int = 3, *,
                  sectors: Optional[Sequence[Tuple[int, int]]] = None, precision: int = PREC_FP64,
-                 tol: float = 1e-10, seed: int = 1234, max_slots: int = 8, ctx: Optional[Context] = None):
+                 tol: float = 1e-10, seed: int = 1234, max_slots: int = 8, ctx: Optional[Context] = None,
+                 reuse_catalog: bool = True):
     """Flux scan (``ed_scan!`` with ``twisted_phases_over_2pi_list``, ``:1320-1355``): every
     (twist, momentum) sector solve is submitted to one continuous-batching server.
 
-    Returns {(theta, k_label): evals}.  The orbit partition does not depend on the twist,
-    but each twist gets its own group phases, catalog and sectors as in the reference."""
+    Returns {(theta, k_label): evals}.  The orbit partition does not depend on the twist:
+    with ``reuse_catalog`` one catalog is built and its phases refreshed per twist
+    (``update_orbit_stabilizer_phases!``, ``:338-369``; the reference's own loop rebuilds it,
+    ``:1348``); every twist's sectors keep the phases they were built with."""
     ctx = ctx or default_context()
     srv = SectorServer(ctx, max_slots=max_slots)
     keep = []
     ids = {}
+    cat = None
     for th in thetas:
         m = twisted(model, th)
         G = model_group(m, th)
         irreps = translation_irreps(G, m.lattice)
-        cat = build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
-                                           particle_statistics=m.statistics, ctx=ctx)
+        if cat is not None and reuse_catalog:
+            cat.update_phases(G)
+        else:
+            cat = build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                               particle_statistics=m.statistics, ctx=ctx)
         for ir in irreps:
             if sectors is not None and tuple(ir.label) not in set(map(tuple, sectors)):
                 continue
@@ -458,7 +482,8 @@ def ed_scan_flux(model: Model, thetas: Sequence[Tuple[float, float]], nev: int =
             b.set_terms(m.bilinear_terms, m.density_terms)
             keep.append(b)
             ids[(tuple(th), tuple(ir.label))] = srv.submit(b, min(nev, b.dim), tol, seed, 300, precision)
-        keep.append(cat)
+        if cat not in keep:
+            keep.append(cat)
     srv.run(0)
     out = {}
     for key, rid in ids.items():

@@ -495,3 +495,55 @@ def test_fock_expansion_and_position_operator_match_oracle(ed, ctx, oracle_mod, 
     gb0.close()
     gb1.close()
     gc.close()
+
+
+@pytest.mark.parametrize("name,th0,th1", [("checker3x2_f_flux", (0.0, 0.0), (0.23, 0.17)),
+                                          ("hald3x2_flux", (0.1, 0.0), (0.4, 0.0)),
+                                          ("checker3x2_b_flux", (0.3, 0.3), (0.0, 0.0))])
+def test_flux_refresh_of_a_device_catalog(ed, ctx, oracle_mod, name, th0, th1):
+    """sed_catalog_update_phases (update_orbit_stabilizer_phases!) on a catalog built at
+    theta0 reproduces the oracle catalog built at theta1 (tables bit-exact, amplitudes and
+    stabilizer phases to 1e-13); a sector built after the refresh solves theta1, one built
+    before it still solves theta0 (1e-10 against the oracle)."""
+    f = CASES[name][0]
+    m0, m1 = L.twisted(f(), th0), L.twisted(f(), th1)
+    G0, G1 = L.model_group(m0, th0), L.model_group(m1, th1)
+    ferm = m0.statistics == L.FERMIONIC
+    print(f"[flux refresh] {name} {th0} -> {th1}", flush=True)
+    gc = ed.build_symmetry_orbit_catalog(n_site=m0.n_site, n_filled=m0.n_filled, symmetry_group=G0,
+                                         particle_statistics=m0.statistics, ctx=ctx)
+    ir0 = L.translation_irreps(G0, m0.lattice)[1]
+    old = ed.build_symmetry_sector_basis(gc, ir0)
+    Hold, nold = ed.hamiltonian_linear_operator(old, m0.bilinear_terms, m0.density_terms)
+    ed.update_orbit_stabilizer_phases(gc, G1)
+    oc1 = oracle_mod.Catalog(m1.n_site, m1.n_filled, G1, ferm)
+    a, b = gc.tables(), oc1.tables()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.abs(a[2] - b[2]).max() < 1e-13
+    ga, ob = gc.orbits(), oc1.orbits()
+    assert np.array_equal(ga[2], ob[2]) and np.abs(ga[3] - ob[3]).max() < 1e-13
+    for mm, G, ir, basis in ((m1, G1, L.translation_irreps(G1, m1.lattice)[1], None), (m0, G0, ir0, old)):
+        if basis is None:
+            basis = ed.build_symmetry_sector_basis(gc, ir)
+            H, n = ed.hamiltonian_linear_operator(basis, mm.bilinear_terms, mm.density_terms)
+        else:
+            H, n = Hold, nold
+        oc = oracle_mod.Catalog(mm.n_site, mm.n_filled, G, ferm)
+        os_ = oracle_mod.Sector(oc, ir.values)
+        os_.set_terms(*L.term_arrays(mm))
+        ref = oracle_mod.lowest_eigenvalues(os_, 3)
+        vals = ed.diagonalize_block_matrixfree(H, n, nev=3, return_vectors=False)[0]
+        assert np.abs(vals - ref).max() <= 1e-10 * np.abs(ref).max()
+        basis.close()
+    gc.close()
+
+
+def test_flux_scan_reusing_one_catalog_matches_rebuilds(ed, ctx):
+    """ed_scan_flux with one catalog refreshed per twist gives the eigenvalues of a catalog
+    rebuilt per twist (the reference's loop), for every (twist, momentum)."""
+    base = L.haldane_honeycomb(3, 2, n_filled=3, V1=0.3)
+    thetas = [(i / 5, 0.1 * i) for i in range(5)]
+    a = ed.ed_scan_flux(base, thetas, nev=2, ctx=ctx, reuse_catalog=True)
+    b = ed.ed_scan_flux(base, thetas, nev=2, ctx=ctx, reuse_catalog=False)
+    assert a.keys() == b.keys() and len(a) == 5 * 6
+    for k in a:
+        assert np.abs(a[k] - b[k]).max() <= 1e-12 * np.abs(b[k]).max(), k

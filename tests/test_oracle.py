@@ -397,3 +397,30 @@ def test_fock_expansion_and_position_operator_against_embedding(oracle_mod, name
         if r > 0:
             dense[r - 1, col] = v
     assert np.abs(dense - P).max() < 1e-12
+
+
+@pytest.mark.parametrize("case", [
+    ("checker3x2_ferm", (0.0, 0.0), (0.23, 0.17)), ("checker3x2_ferm", (0.4, 0.1), (0.0, 0.0)),
+    ("hald2x3", (0.1, 0.0), (0.35, 0.2)), ("hub2x3", (0.0, 0.0), (0.31, 0.0)),
+])
+def test_flux_refresh_equals_rebuild(oracle_mod, case):
+    """update_orbit_stabilizer_phases! from twist theta0 to theta1 gives the catalog built
+    at theta1 directly: rep_rank / rep_sym / stabilizer indices bit-exact, rep_amp and
+    stabilizer phases to 1e-14 (the twist changes only phases, not the orbit partition)."""
+    name, th0, th1 = case
+    base = {"checker3x2_ferm": lambda: L.checkerboard(3, 2, n_filled=4, V1=1.0, V2=0.5),
+            "hald2x3": lambda: L.haldane_honeycomb(2, 3, n_filled=3),
+            "hub2x3": lambda: L.hubbard_square(2, 3, U=4.0, n_filled=5)}[name]()
+    m0, m1 = L.twisted(base, th0), L.twisted(base, th1)
+    G0, G1 = L.model_group(m0, th0), L.model_group(m1, th1)
+    ferm = base.statistics == L.FERMIONIC
+    upd = oracle_mod.Catalog(base.n_site, base.n_filled, G0, ferm).update_phases(G1)
+    ref = oracle_mod.Catalog(base.n_site, base.n_filled, G1, ferm)
+    a, b = upd.tables(), ref.tables()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.abs(a[2] - b[2]).max() < 1e-14
+    oa, ob = upd.orbits(), ref.orbits()
+    assert np.array_equal(oa[0], ob[0]) and np.array_equal(oa[1], ob[1]) and np.array_equal(oa[2], ob[2])
+    assert np.abs(oa[3] - ob[3]).max() < 1e-14
+    if th0 != th1:
+        assert np.abs(a[2] - oracle_mod.Catalog(base.n_site, base.n_filled, G0, ferm).tables()[2]).max() > 1e-3
