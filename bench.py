@@ -311,6 +311,40 @@ def serving_stream(E, L, ctx, n_req=32, seed=2024, nev=3, lanes=8, load=1.2):
     }
 
 
+def solve_model(L, name):
+    """A benchmark workload by name (see lattice.workload), or a serving-stream family:
+    xxzN (XXZ chain, N sites, half filling), ladderL (2 x L Hubbard ladder, quarter filling)."""
+    if name.startswith("xxz") and name[3:].isdigit() and name != "xxz16":
+        return L.heisenberg_chain(int(name[3:]))
+    if name.startswith("ladder") and name[6:].isdigit():
+        return L.hubbard_square(int(name[6:]), 2, U=4.0, t=1.0, n_filled=int(name[6:]))
+    return L.workload(name)
+
+
+def single_solves(args):
+    """One sector solve per precision (plain FP64, then FP32 + FP64 refinement), each
+    repeated; prints one JSON line per solve with its Lanczos step counts and time."""
+    from realspace_exactdiagonalization_amd import ed as E
+    from realspace_exactdiagonalization_amd import lattice as L
+    m = solve_model(L, args.solve)
+    G = L.model_group(m)
+    ctx = E.Context(0)
+    cat = E.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    ir = L.translation_irreps(G, m.lattice)[args.irrep]
+    b = E.build_symmetry_sector_basis(cat, ir)
+    H, n = E.hamiltonian_linear_operator(b, m.bilinear_terms, m.density_terms)
+    for prec, label in ((E.PREC_FP64, "fp64"), (E.PREC_FP32_REFINE, "fp32_refine")):
+        for rep in range(args.solve_repeats):
+            st = {}
+            t0 = time.time()
+            vals, _ = E.diagonalize_block_matrixfree(H, n, nev=args.nev, precision=prec, return_vectors=False,
+                                                     stats=st)
+            print(json.dumps({"solve": args.solve, "irrep": list(ir.label), "dim": n, "precision": label,
+                              "repeat": rep, "wall_s": round(time.time() - t0, 4), "stats": st,
+                              "evals": vals.tolist()}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,12 +357,21 @@ def main():
     ap.add_argument("--lanes", type=int, default=8, help="server lanes for the flux scan and the stream")
     ap.add_argument("--no-stream", action="store_true", help="skip the serving-stream benchmark")
     ap.add_argument("--stream-requests", type=int, default=32)
+    ap.add_argument("--solve", default=None,
+                    help="only time sector solves of this model (a workload name, xxzN or ladderL)")
+    ap.add_argument("--irrep", type=int, default=0, help="momentum sector index for --solve")
+    ap.add_argument("--nev", type=int, default=3, help="eigenpairs for --solve")
+    ap.add_argument("--solve-repeats", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+
+    if args.solve:
+        single_solves(args)
+        return
 
     if args.copy_calibration:
         import torch
