@@ -131,7 +131,8 @@ int sed_matvec_device(sed_sector* sec, const void* x_dev, void* y_dev, int preci
 int sed_matvec_benchmark(sed_sector* sec, int precision, int n_calls, double* ms_per_call);
 /* Launch shape of the H|psi> kernel (no reference counterpart; the reference has one
  * threaded loop).  Shapes are indexed 0..n_shapes-1; each has `lanes` lanes per row
- * (64/lanes rows per wavefront) and `batch` hops in flight per lane.  xcd_chunk is the
+ * (64/lanes rows per wavefront), `batch` hops in flight per lane and a workgroup size
+ * (128, 256 or 512 threads; a shape whose LDS does not fit is skipped).  xcd_chunk is the
  * run length in which each of the 8 XCDs takes consecutive row blocks (1 = natural
  * order, 0 = one contiguous range per XCD).  shape = -1 re-enables the autotuner
  * (xcd_chunk ignored), which on the first matvec of a sector of dimension >= 2^17 times
@@ -140,12 +141,12 @@ int sed_matvec_benchmark(sed_sector* sec, int precision, int n_calls, double* ms
  * length 1). */
 int sed_sector_set_shape(sed_sector* sec, int precision, int shape, int xcd_chunk);
 /* Chosen shape (-1 if not chosen yet), its lanes/batch and XCD run length, the numbers
- * of shapes and of run lengths tried, and the autotuner's mean times in ms (may be
- * NULL; 2 * n_shapes + n_chunks floats: every shape at run length 1, every run length
- * for the fastest of those, every shape again at the fastest run length; 0 if not
- * timed). */
+ * of shapes and of run lengths tried, the autotuner's times in ms (may be NULL;
+ * 2 * n_shapes + n_chunks floats: every shape at run length 1, every run length for the
+ * fastest of those, every shape again at the fastest run length; 0 if not timed, -1 if
+ * the shape does not fit), and the shape's workgroup size (may be NULL). */
 int sed_sector_shape(const sed_sector* sec, int precision, int* shape, int* lanes, int* batch, int* xcd_chunk,
-                     int* n_shapes, int* n_chunks, float* tune_ms);
+                     int* n_shapes, int* n_chunks, float* tune_ms, int* workgroup);
 /* diagonalize_block_matrixfree(H_op, n; nev) (:998-1030): lowest nev eigenvalues
  * (ascending) and optionally eigenvectors (sector_dim x nev column-major ComplexF64,
  * may be NULL).  Sectors of dim <= 500 are solved densely, as the reference does;

@@ -80,7 +80,7 @@ SIGNATURES = {
     "sed_matvec_benchmark": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "sed_sector_set_shape": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "sed_sector_shape": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                   C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _vp, C.POINTER(C.c_int)]),
     "sed_solve": (C.c_int, [_vp, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int, _f64p, _vp,
                             C.POINTER(SolveStats)]),
     "sed_sector_density": (C.c_int, [_vp, C.c_int, _f64p]),

@@ -36,7 +36,7 @@ int launch_convert(const double* a, float* b, int64_t n, hipStream_t s);
 int launch_convert(const float* a, double* b, int64_t n, hipStream_t s);
 int launch_matvec_xhat(sed_sector* sec, const void* xhat, void* y, bool fp64, hipStream_t s);
 int matvec_shape_info(const sed_sector* sec, int precision, int* lanes, int* batch, int* chunk, float* tune_ms,
-                      int* n_chunks);
+                      int* n_chunks, int* block);
 
 template <typename T>
 static int upload(DevBuf<T>& b, const T* h, size_t n, hipStream_t s) {
@@ -627,7 +627,7 @@ int sed_matvec_benchmark(sed_sector* sec, int precision, int n_calls, double* ms
 int sed_sector_set_shape(sed_sector* sec, int precision, int shape, int xcd_chunk) {
     if (!sec) return fail(SED_EINVAL, "NULL sector");
     if (precision != SED_PREC_FP64 && precision != SED_PREC_FP32) return fail(SED_EINVAL, "bad precision");
-    const int n = matvec_shape_info(sec, precision, nullptr, nullptr, nullptr, nullptr, nullptr);
+    const int n = matvec_shape_info(sec, precision, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (shape < -1 || shape >= n) return fail(SED_EINVAL, "shape out of range");
     if (shape >= 0 && xcd_chunk < 0) return fail(SED_EINVAL, "xcd_chunk must be >= 0");
     const int pi = precision == SED_PREC_FP64 ? 0 : 1;
@@ -637,10 +637,10 @@ int sed_sector_set_shape(sed_sector* sec, int precision, int shape, int xcd_chun
 }
 
 int sed_sector_shape(const sed_sector* sec, int precision, int* shape, int* lanes, int* batch, int* xcd_chunk,
-                     int* n_shapes, int* n_chunks, float* tune_ms) {
+                     int* n_shapes, int* n_chunks, float* tune_ms, int* workgroup) {
     if (!sec) return fail(SED_EINVAL, "NULL sector");
     if (precision != SED_PREC_FP64 && precision != SED_PREC_FP32) return fail(SED_EINVAL, "bad precision");
-    const int n = matvec_shape_info(sec, precision, lanes, batch, xcd_chunk, tune_ms, n_chunks);
+    const int n = matvec_shape_info(sec, precision, lanes, batch, xcd_chunk, tune_ms, n_chunks, workgroup);
     if (shape) *shape = sec->mv_shape[precision == SED_PREC_FP64 ? 0 : 1];
     if (n_shapes) *n_shapes = n;
     return SED_OK;

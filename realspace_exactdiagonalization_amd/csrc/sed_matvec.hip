@@ -44,16 +44,16 @@ namespace sed {
 // fixed xor butterfly over the row's lanes.  Hops are processed BATCH at a time: BATCH
 // independent table reads are issued before the BATCH dependent gathers, so each lane
 // keeps several memory requests in flight.
-template <typename M, typename R, bool FERMI, int LPR, int BATCH>
-__global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* __restrict__ rep,
-                                                     const uint8_t* __restrict__ stab,
-                                                     const uint32_t* __restrict__ table,
-                                                     const typename C2<R>::T* __restrict__ xhat,
-                                                     typename C2<R>::T* __restrict__ y, int64_t nblocks,
-                                                     int64_t chunk) {
+template <typename M, typename R, bool FERMI, int LPR, int BATCH, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_matvec(MvConst c, const uint64_t* __restrict__ rep,
+                                                  const uint8_t* __restrict__ stab,
+                                                  const uint32_t* __restrict__ table,
+                                                  const typename C2<R>::T* __restrict__ xhat,
+                                                  typename C2<R>::T* __restrict__ y, int64_t nblocks,
+                                                  int64_t chunk) {
     using T = typename C2<R>::T;
-    constexpr int RPB = kMvBlock / LPR;
-    const MvLds<R> L = stage_mv<R>(c, kMvBlock);
+    constexpr int RPB = BLOCK / LPR;
+    const MvLds<R> L = stage_mv<R>(c, BLOCK);
     const int64_t lb = xcd_block(nblocks, chunk);
     if (lb < 0) return;
     const int sub = threadIdx.x % LPR;
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* 
     const bool live = r < c.dim;
     const M m = live ? (M)rep[r] : (M)0;
     R ax = 0, ay = 0, dx = 0, dy = 0;
-    row_sums<M, R, FERMI, LPR, BATCH, kMvBlock>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx, dy);
+    row_sums<M, R, FERMI, LPR, BATCH, BLOCK>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx, dy);
     if (LPR > 1) {
 #pragma unroll
         for (int off = LPR / 2; off >= 1; off >>= 1) {
@@ -82,11 +82,13 @@ __global__ void __launch_bounds__(kMvBlock) k_matvec(MvConst c, const uint64_t* 
 }
 
 // Launch shapes the autotuner chooses from: lanes per row (rows per wavefront =
-// 64 / lanes) and hops per batch.
+// 64 / lanes), hops per batch and workgroup size.  A shape whose LDS (rank scratch of
+// k entries per thread) does not fit a workgroup is skipped.
 struct MvShape {
-    int lanes, batch;
+    int lanes, batch, block;
 };
-static const MvShape kShapes[] = {{1, 1}, {1, 4}, {2, 4}, {4, 4}, {8, 4}, {4, 2}};
+static const MvShape kShapes[] = {{1, 1, 256}, {1, 4, 256}, {2, 4, 256}, {4, 4, 256}, {8, 4, 256},
+                                  {4, 2, 256}, {1, 4, 128}, {2, 4, 128}, {1, 4, 512}, {2, 4, 512}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // XCD run lengths the autotuner tries for the chosen shape (0 = one range per XCD)
 static const int kChunks[] = {1, 16, 64, 256, 1024, 0};
@@ -269,15 +271,18 @@ size_t matvec_lds_bytes(const sed_sector* sec) {
 template size_t matvec_lds_bytes<float>(const sed_sector*);
 template size_t matvec_lds_bytes<double>(const sed_sector*);
 
-template <typename M, typename R, bool F, int LPR, int B>
-static int launch_shape(const MvConst& c, size_t lds, sed_sector* sec, const typename C2<R>::T* xh,
-                        typename C2<R>::T* y, int chunk, hipStream_t s) {
-    const int64_t rpb = kMvBlock / LPR;
+template <typename M, typename R, bool F, int LPR, int B, int BLOCK>
+static int launch_shape(const MvConst& c, sed_sector* sec, const typename C2<R>::T* xh, typename C2<R>::T* y,
+                        int chunk, hipStream_t s) {
+    size_t off[9];
+    const size_t lds = mv_lds_layout<R>(c, off, BLOCK);
+    if ((int)lds > sec->cat->ctx->max_lds) return fail(SED_EUNSUPPORTED, "launch shape needs too much LDS");
+    const int64_t rpb = BLOCK / LPR;
     const int64_t nb = (sec->dim + rpb - 1) / rpb;
     const int64_t ch = chunk > 0 ? chunk : std::max<int64_t>(1, (nb + 7) / 8);
     const int64_t grid = ((nb + 8 * ch - 1) / (8 * ch)) * (8 * ch);
-    hipLaunchKernelGGL((k_matvec<M, R, F, LPR, B>), dim3((unsigned)grid), dim3(kMvBlock), lds, s, c, sec->d_rep.p,
-                       sec->d_stab.p, sec->d_table.p, xh, y, nb, ch);
+    hipLaunchKernelGGL((k_matvec<M, R, F, LPR, B, BLOCK>), dim3((unsigned)grid), dim3(BLOCK), lds, s, c,
+                       sec->d_rep.p, sec->d_stab.p, sec->d_table.p, xh, y, nb, ch);
     SED_HIP(hipGetLastError());
     return SED_OK;
 }
@@ -286,16 +291,26 @@ template <typename M, typename R, bool F>
 static int run_shape(sed_sector* sec, int shape, int chunk, const typename C2<R>::T* xh, typename C2<R>::T* y,
                      hipStream_t s) {
     MvConst c = make_const<R>(sec);
-    size_t off[9];
-    const size_t lds = mv_lds_layout<R>(c, off, kMvBlock);
     switch (shape) {
-        case 0: return launch_shape<M, R, F, 1, 1>(c, lds, sec, xh, y, chunk, s);
-        case 1: return launch_shape<M, R, F, 1, 4>(c, lds, sec, xh, y, chunk, s);
-        case 2: return launch_shape<M, R, F, 2, 4>(c, lds, sec, xh, y, chunk, s);
-        case 3: return launch_shape<M, R, F, 4, 4>(c, lds, sec, xh, y, chunk, s);
-        case 4: return launch_shape<M, R, F, 8, 4>(c, lds, sec, xh, y, chunk, s);
-        default: return launch_shape<M, R, F, 4, 2>(c, lds, sec, xh, y, chunk, s);
+        case 0: return launch_shape<M, R, F, 1, 1, 256>(c, sec, xh, y, chunk, s);
+        case 1: return launch_shape<M, R, F, 1, 4, 256>(c, sec, xh, y, chunk, s);
+        case 2: return launch_shape<M, R, F, 2, 4, 256>(c, sec, xh, y, chunk, s);
+        case 3: return launch_shape<M, R, F, 4, 4, 256>(c, sec, xh, y, chunk, s);
+        case 4: return launch_shape<M, R, F, 8, 4, 256>(c, sec, xh, y, chunk, s);
+        case 5: return launch_shape<M, R, F, 4, 2, 256>(c, sec, xh, y, chunk, s);
+        case 6: return launch_shape<M, R, F, 1, 4, 128>(c, sec, xh, y, chunk, s);
+        case 7: return launch_shape<M, R, F, 2, 4, 128>(c, sec, xh, y, chunk, s);
+        case 8: return launch_shape<M, R, F, 1, 4, 512>(c, sec, xh, y, chunk, s);
+        default: return launch_shape<M, R, F, 2, 4, 512>(c, sec, xh, y, chunk, s);
     }
+}
+
+// LDS of a shape's workgroup fits the device
+template <typename R>
+static bool shape_fits(const sed_sector* sec, int shape) {
+    MvConst c = make_const<R>(sec);
+    size_t off[9];
+    return (int)mv_lds_layout<R>(c, off, kShapes[shape].block) <= sec->cat->ctx->max_lds;
 }
 
 template <typename R>
@@ -373,6 +388,10 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<
     int best_i = 1, best_c = 1;
     for (int i = 0; i < kNumShapes; ++i) {
         float ms = 0;
+        if (!shape_fits<R>(sec, i)) {
+            sec->tune_ms[pi][i] = -1.0f;
+            continue;
+        }
         SED_TRY(time_shape<R>(sec, i, 1, xh, y, s, e0, e1, &ms));
         sec->tune_ms[pi][i] = ms;
         if (ms < best) {
@@ -394,6 +413,10 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<
     const int shape_c = best_i;
     for (int i = 0; i < kNumShapes; ++i) {
         float ms = 0;
+        if (!shape_fits<R>(sec, i)) {
+            sec->tune_ms[pi][kNumShapes + kNumChunks + i] = -1.0f;
+            continue;
+        }
         if (i == shape_c) ms = sec->tune_ms[pi][kNumShapes + (int)(std::find(kChunks, kChunks + kNumChunks, best_c) - kChunks)];
         else SED_TRY(time_shape<R>(sec, i, best_c, xh, y, s, e0, e1, &ms));
         sec->tune_ms[pi][kNumShapes + kNumChunks + i] = ms;
@@ -420,11 +443,12 @@ static int run_mv(sed_sector* sec, const typename C2<R>::T* xh, typename C2<R>::
 }
 
 int matvec_shape_info(const sed_sector* sec, int precision, int* lanes, int* batch, int* chunk, float* tune_ms,
-                      int* n_chunks) {
+                      int* n_chunks, int* block) {
     const int pi = precision == SED_PREC_FP64 ? 0 : 1;
     const int sh = sec->mv_shape[pi];
     if (lanes) *lanes = sh >= 0 ? kShapes[sh].lanes : 0;
     if (batch) *batch = sh >= 0 ? kShapes[sh].batch : 0;
+    if (block) *block = sh >= 0 ? kShapes[sh].block : 0;
     if (chunk) *chunk = sec->mv_chunk[pi];
     if (n_chunks) *n_chunks = kNumChunks;
     if (tune_ms)

@@ -199,11 +199,12 @@ class SymmetrySectorBasis:
         check(lib().sed_sector_set_shape(self._h, precision, shape, xcd_chunk))
 
     def shape(self, precision: int) -> dict:
-        sh, ln, bt, ch, ns, nc = (C.c_int() for _ in range(6))
+        sh, ln, bt, ch, ns, nc, wg = (C.c_int() for _ in range(7))
         ms = np.zeros(32, dtype=np.float32)
         check(lib().sed_sector_shape(self._h, precision, C.byref(sh), C.byref(ln), C.byref(bt), C.byref(ch),
-                                     C.byref(ns), C.byref(nc), ptr(ms)))
-        return {"shape": sh.value, "lanes_per_row": ln.value, "batch": bt.value, "xcd_chunk": ch.value,
+                                     C.byref(ns), C.byref(nc), ptr(ms), C.byref(wg)))
+        return {"shape": sh.value, "lanes_per_row": ln.value, "batch": bt.value, "workgroup": wg.value,
+                "xcd_chunk": ch.value,
                 "tune_ms": [round(float(v), 4) for v in ms[:ns.value]],
                 "tune_ms_xcd_chunk": [round(float(v), 4) for v in ms[ns.value:ns.value + nc.value]],
                 "tune_ms_at_chunk": [round(float(v), 4) for v in ms[ns.value + nc.value:2 * ns.value + nc.value]]}
