@@ -389,7 +389,7 @@ def main():
     from realspace_exactdiagonalization_amd import lattice as L
 
     t_setup = time.time()
-    model = L.workload(args.workload)
+    model = solve_model(L, args.workload)
     G = L.model_group(model)
     ir = L.translation_irreps(G, model.lattice)[0]
     ctx = E.Context(0)

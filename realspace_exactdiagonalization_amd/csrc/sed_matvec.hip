@@ -44,7 +44,7 @@ namespace sed {
 // fixed xor butterfly over the row's lanes.  Hops are processed BATCH at a time: BATCH
 // independent table reads are issued before the BATCH dependent gathers, so each lane
 // keeps several memory requests in flight.
-template <typename M, typename R, bool FERMI, int LPR, int BATCH, int BLOCK>
+template <typename M, typename R, bool FERMI, int LPR, int BATCH, int BLOCK, bool FLAT>
 __global__ void __launch_bounds__(BLOCK) k_matvec(MvConst c, const uint64_t* __restrict__ rep,
                                                   const uint8_t* __restrict__ stab,
                                                   const uint32_t* __restrict__ table,
@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(BLOCK) k_matvec(MvConst c, const uint64_t* __r
                                                   int64_t chunk) {
     using T = typename C2<R>::T;
     constexpr int RPB = BLOCK / LPR;
-    const MvLds<R> L = stage_mv<R>(c, BLOCK);
+    const MvLds<R> L = stage_mv<R>(c, BLOCK, FLAT);
     const int64_t lb = xcd_block(nblocks, chunk);
     if (lb < 0) return;
     const int sub = threadIdx.x % LPR;
@@ -61,7 +61,10 @@ __global__ void __launch_bounds__(BLOCK) k_matvec(MvConst c, const uint64_t* __r
     const bool live = r < c.dim;
     const M m = live ? (M)rep[r] : (M)0;
     R ax = 0, ay = 0, dx = 0, dy = 0;
-    row_sums<M, R, FERMI, LPR, BATCH, BLOCK>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx, dy);
+    if (FLAT)
+        row_sums_flat<M, R, FERMI, LPR, BATCH, BLOCK>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx, dy);
+    else
+        row_sums<M, R, FERMI, LPR, BATCH, BLOCK>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx, dy);
     if (LPR > 1) {
 #pragma unroll
         for (int off = LPR / 2; off >= 1; off >>= 1) {
@@ -84,11 +87,17 @@ __global__ void __launch_bounds__(BLOCK) k_matvec(MvConst c, const uint64_t* __r
 // Launch shapes the autotuner chooses from: lanes per row (rows per wavefront =
 // 64 / lanes), hops per batch and workgroup size.  A shape whose LDS (rank scratch of
 // k entries per thread) does not fit a workgroup is skipped.
+// Shapes with `flat` walk each lane's hops in batches across site boundaries
+// (row_sums_flat), the others site by site (row_sums).
 struct MvShape {
     int lanes, batch, block;
+    bool flat;
 };
-static const MvShape kShapes[] = {{1, 1, 256}, {1, 4, 256}, {2, 4, 256}, {4, 4, 256}, {8, 4, 256},
-                                  {4, 2, 256}, {1, 4, 128}, {2, 4, 128}, {1, 4, 512}, {2, 4, 512}};
+static const MvShape kShapes[] = {{1, 1, 256, false}, {1, 4, 256, false}, {2, 4, 256, false},
+                                  {4, 4, 256, false}, {8, 4, 256, false}, {4, 2, 256, false},
+                                  {2, 4, 128, false}, {1, 4, 512, false}, {2, 4, 512, false},
+                                  {1, 4, 256, true},  {1, 8, 256, true},  {2, 4, 256, true},
+                                  {2, 8, 256, true}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // XCD run lengths the autotuner tries for the chosen shape (0 = one range per XCD)
 static const int kChunks[] = {1, 16, 64, 256, 1024, 0};
@@ -271,17 +280,26 @@ size_t matvec_lds_bytes(const sed_sector* sec) {
 template size_t matvec_lds_bytes<float>(const sed_sector*);
 template size_t matvec_lds_bytes<double>(const sed_sector*);
 
-template <typename M, typename R, bool F, int LPR, int B, int BLOCK>
+// The flat hop loop's prefix sums need C(n-1, k+1) below 2^32 (the binomial table
+// saturates there).
+static bool flat_ranks_exact(const sed_sector* sec) {
+    const sed_catalog* cat = sec->cat;
+    if (cat->n_site < 1) return false;
+    return cat->binom[(size_t)(cat->n_site - 1) * cat->bstride + cat->n_filled + 1] != 0xffffffffu;
+}
+
+template <typename M, typename R, bool F, int LPR, int B, int BLOCK, bool FLAT = false>
 static int launch_shape(const MvConst& c, sed_sector* sec, const typename C2<R>::T* xh, typename C2<R>::T* y,
                         int chunk, hipStream_t s) {
     size_t off[9];
-    const size_t lds = mv_lds_layout<R>(c, off, BLOCK);
+    const size_t lds = mv_lds_layout<R>(c, off, BLOCK, FLAT);
     if ((int)lds > sec->cat->ctx->max_lds) return fail(SED_EUNSUPPORTED, "launch shape needs too much LDS");
+    if (FLAT && !flat_ranks_exact(sec)) return fail(SED_EUNSUPPORTED, "flat hop loop: C(n-1, k+1) >= 2^32");
     const int64_t rpb = BLOCK / LPR;
     const int64_t nb = (sec->dim + rpb - 1) / rpb;
     const int64_t ch = chunk > 0 ? chunk : std::max<int64_t>(1, (nb + 7) / 8);
     const int64_t grid = ((nb + 8 * ch - 1) / (8 * ch)) * (8 * ch);
-    hipLaunchKernelGGL((k_matvec<M, R, F, LPR, B, BLOCK>), dim3((unsigned)grid), dim3(BLOCK), lds, s, c,
+    hipLaunchKernelGGL((k_matvec<M, R, F, LPR, B, BLOCK, FLAT>), dim3((unsigned)grid), dim3(BLOCK), lds, s, c,
                        sec->d_rep.p, sec->d_stab.p, sec->d_table.p, xh, y, nb, ch);
     SED_HIP(hipGetLastError());
     return SED_OK;
@@ -298,19 +316,23 @@ static int run_shape(sed_sector* sec, int shape, int chunk, const typename C2<R>
         case 3: return launch_shape<M, R, F, 4, 4, 256>(c, sec, xh, y, chunk, s);
         case 4: return launch_shape<M, R, F, 8, 4, 256>(c, sec, xh, y, chunk, s);
         case 5: return launch_shape<M, R, F, 4, 2, 256>(c, sec, xh, y, chunk, s);
-        case 6: return launch_shape<M, R, F, 1, 4, 128>(c, sec, xh, y, chunk, s);
-        case 7: return launch_shape<M, R, F, 2, 4, 128>(c, sec, xh, y, chunk, s);
-        case 8: return launch_shape<M, R, F, 1, 4, 512>(c, sec, xh, y, chunk, s);
-        default: return launch_shape<M, R, F, 2, 4, 512>(c, sec, xh, y, chunk, s);
+        case 6: return launch_shape<M, R, F, 2, 4, 128>(c, sec, xh, y, chunk, s);
+        case 7: return launch_shape<M, R, F, 1, 4, 512>(c, sec, xh, y, chunk, s);
+        case 8: return launch_shape<M, R, F, 2, 4, 512>(c, sec, xh, y, chunk, s);
+        case 9: return launch_shape<M, R, F, 1, 4, 256, true>(c, sec, xh, y, chunk, s);
+        case 10: return launch_shape<M, R, F, 1, 8, 256, true>(c, sec, xh, y, chunk, s);
+        case 11: return launch_shape<M, R, F, 2, 4, 256, true>(c, sec, xh, y, chunk, s);
+        default: return launch_shape<M, R, F, 2, 8, 256, true>(c, sec, xh, y, chunk, s);
     }
 }
 
-// LDS of a shape's workgroup fits the device
+// A shape's workgroup LDS fits the device (and the flat loop's ranks are exact)
 template <typename R>
 static bool shape_fits(const sed_sector* sec, int shape) {
     MvConst c = make_const<R>(sec);
     size_t off[9];
-    return (int)mv_lds_layout<R>(c, off, kShapes[shape].block) <= sec->cat->ctx->max_lds;
+    if (kShapes[shape].flat && !flat_ranks_exact(sec)) return false;
+    return (int)mv_lds_layout<R>(c, off, kShapes[shape].block, kShapes[shape].flat) <= sec->cat->ctx->max_lds;
 }
 
 template <typename R>

@@ -38,8 +38,14 @@ struct MvLds {
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
+// Rank scratch entries per thread: k (per-site S of row_sums) or 2 (k + 1) (the row's
+// prefix arrays of row_sums_flat).
+__host__ __device__ inline int mv_scratch(const MvConst& c, bool flat) {
+    return flat ? 2 * (c.k + 1) : (c.k > 0 ? c.k : 1);
+}
+
 template <typename R>
-__host__ __device__ inline size_t mv_lds_layout(const MvConst& c, size_t* off, int block) {
+__host__ __device__ inline size_t mv_lds_layout(const MvConst& c, size_t* off, int block, bool flat = false) {
     using T = typename C2<R>::T;
     size_t o = 0;
     off[0] = o; o = align16(o + (size_t)(c.n + 1) * c.bs * 4);       // bin
@@ -50,16 +56,16 @@ __host__ __device__ inline size_t mv_lds_layout(const MvConst& c, size_t* off, i
     off[5] = o; o = align16(o + (size_t)c.nG * sizeof(T));            // chi
     off[6] = o; o = align16(o + (size_t)c.n_terms);                   // tto
     off[7] = o; o = align16(o + (size_t)c.n_dens);                    // dj
-    off[8] = o; o = align16(o + (size_t)(c.k > 0 ? c.k : 1) * block * 4);    // S
+    off[8] = o; o = align16(o + (size_t)mv_scratch(c, flat) * block * 4);   // S
     return o;
 }
 
 template <typename R>
-__device__ __forceinline__ MvLds<R> stage_mv(const MvConst& c, int block) {
+__device__ __forceinline__ MvLds<R> stage_mv(const MvConst& c, int block, bool flat = false) {
     using T = typename C2<R>::T;
     extern __shared__ __align__(16) uint8_t smem[];
     size_t off[9];
-    mv_lds_layout<R>(c, off, block);
+    mv_lds_layout<R>(c, off, block, flat);
     MvLds<R> L;
     L.bin = reinterpret_cast<uint32_t*>(smem + off[0]);
     L.tstart = reinterpret_cast<int32_t*>(smem + off[1]);
@@ -176,6 +182,116 @@ __device__ __forceinline__ void row_sums(const MvLds<R>& L, const MvConst& c, co
             }
         }
         ia += LPR;
+    }
+}
+
+// The same sums with every hop of the lane independent.  For m with bits
+// q_0 < ... < q_{k-1}, R(m) = sum_j C(q_j, j+1) and the prefix sums (per row, in the
+// scratch: PD at P[0..k], PU at P[k+1..2k+1], stride SST)
+//   PD[j] = sum_{i<j} (C(q_i, i) - C(q_i, i+1)),   PU[j] = sum_{i<j} (C(q_i, i+2) - C(q_i, i+1)),
+// the hop a = q_ia -> b (b empty; i_b = number of bits of m below b other than a) lands on
+//   rank = R - C(a, i_a+1) + C(b, i_b+1) + (a < b ? PD[i_b+1] - PD[i_a+1] : PU[i_a] - PU[i_b])
+// (the bits between a and b move down or up one index), in uint32 arithmetic modulo 2^32
+// (the host enables this loop only when C(n-1, k+1) < 2^32).  The lane's candidates
+// (sites sub, sub+LPR, ... and their term lists, in order) are walked in batches of
+// BATCH that run across site boundaries: BATCH table reads, then BATCH gathers.
+template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST>
+__device__ __forceinline__ void row_sums_flat(const MvLds<R>& L, const MvConst& c, const M m, const int sub,
+                                              uint32_t* P, const uint32_t* __restrict__ table,
+                                              const typename C2<R>::T* __restrict__ xhat, R& ax, R& ay, R& dx,
+                                              R& dy) {
+    using T = typename C2<R>::T;
+    const int K1 = c.k + 1;
+    uint32_t Rm = 0;
+    {
+        uint32_t pd = 0, pu = 0;
+        M t = m;
+        int j = 0;
+        while (t) {
+            const int q = ctz_m<M>(t);
+            t &= t - 1;
+            const uint32_t c0 = L.bin[q * c.bs + j], c1 = L.bin[q * c.bs + j + 1], c2 = L.bin[q * c.bs + j + 2];
+            P[j * SST] = pd;
+            P[(K1 + j) * SST] = pu;
+            Rm += c1;
+            pd += c0 - c1;
+            pu += c2 - c1;
+            ++j;
+        }
+        P[j * SST] = pd;
+        P[(K1 + j) * SST] = pu;
+    }
+    // this lane's first site
+    M rest = m;
+    for (int i = 0; i < sub && rest; ++i) rest &= rest - 1;
+    int ia = sub;
+    int a = -1, t = 0, t1 = 0;
+    uint32_t ca = 0;
+    auto next_site = [&]() {
+        while (rest) {
+            a = ctz_m<M>(rest);
+            for (int i = 0; i < LPR && rest; ++i) rest &= rest - 1;
+            for (int d = L.dstart[a]; d < L.dstart[a + 1]; ++d) {
+                if ((m >> L.dj[d]) & 1) {
+                    dx += L.dv[d].x;
+                    dy += L.dv[d].y;
+                }
+            }
+            t = L.tstart[a];
+            t1 = L.tstart[a + 1];
+            ca = L.bin[a * c.bs + ia + 1];
+            if (t < t1) return;
+            ia += LPR;
+        }
+        a = -1;
+    };
+    next_site();
+    while (a >= 0) {
+        uint32_t idx[BATCH];
+        bool ok[BATCH];
+        int tt[BATCH], pj[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+            if (a >= 0 && t >= t1) {
+                ia += LPR;
+                next_site();
+            }
+            const bool live = a >= 0;
+            const int tc = live ? t : 0;
+            const int b = L.tto[tc];
+            const int below = popc_m<M>(m & (((M)1 << b) - 1));
+            const int ib = below - ((live && a < b) ? 1 : 0);
+            const uint32_t mid = (live && a < b) ? P[(ib + 1) * SST] - P[(ia + 1) * SST]
+                                                 : P[(K1 + (live ? ia : 0)) * SST] - P[(K1 + ib) * SST];
+            ok[u] = live && !((m >> b) & 1);
+            idx[u] = Rm - ca + L.bin[b * c.bs + ib + 1] + mid;
+            tt[u] = tc;
+            pj[u] = FERMI ? ((ib - ia) & 1) : 0;
+            if (live) ++t;
+        }
+        uint32_t e[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+            const uint32_t v = table[ok[u] ? idx[u] : 0u];
+            e[u] = ok[u] ? v : kNoRow;
+        }
+        T v[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) v[u] = xhat[e[u] != kNoRow ? (e[u] >> c.shift) : 0u];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+            const bool live = e[u] != kNoRow;
+            const int g = live ? (int)((e[u] >> 1) & c.gmask) : 0;
+            const T cf = cmul(L.tcoef[tt[u]], L.chi[g]);
+            const T pr = cmul(cf, v[u]);
+            const R sg = live ? ((((e[u] & 1) ^ (uint32_t)pj[u]) != 0) ? (R)-1 : (R)1) : (R)0;
+            ax += sg * pr.x;
+            ay += sg * pr.y;
+        }
+        if (a >= 0 && t >= t1) {
+            ia += LPR;
+            next_site();
+        }
     }
 }
 
