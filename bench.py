@@ -26,6 +26,11 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
+def log(msg, **_):
+    """Progress lines go to stderr; stdout carries only the result JSON line."""
+    print(msg, file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes(dim, gated, ins, s):
     """Algorithmic bytes of one sector matvec with the per-unit figures of the workload
     definition: per row the representative (8 B), its norm (4 B), x[row] and the y[row]
@@ -91,7 +96,7 @@ def cpu_baseline(model, G, ir, cat, basis, seconds, threads, bytes_c64, gated, i
     t_sample = time.time() - t0
     t_full = t_fixed + max(t_sample - t_fixed, 0.0) * D / cols
     gbs = bytes_c64 / t_full / 1e9
-    print(f"[cpu] prep {prep:.1f}s fixed {t_fixed:.2f}s sample {cols} cols {t_sample:.2f}s -> "
+    log(f"[cpu] prep {prep:.1f}s fixed {t_fixed:.2f}s sample {cols} cols {t_sample:.2f}s -> "
           f"{t_full:.2f} s/matvec {gbs:.2f} GB/s", flush=True)
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": f"on-the-fly matvec (FP64) over columns [0,{cols}) of {D} "
@@ -403,7 +408,7 @@ def main():
     H = E.MatrixFreeHamiltonian(basis, model.bilinear_terms, model.density_terms)
     gated, ins = basis.hop_counts()
     dim = basis.dim
-    print(f"[bench] {args.workload}: C(N,k)={cat.n_total} orbits={cat.n_orbits} dim={dim} "
+    log(f"[bench] {args.workload}: C(N,k)={cat.n_total} orbits={cat.n_orbits} dim={dim} "
           f"gated_hops={gated} in_sector_hops={ins} catalog={t_cat:.2f}s sector={t_sec:.2f}s", flush=True)
 
     res = {}
@@ -417,21 +422,21 @@ def main():
         Bl = layout_bytes(dim, gated, ins, s)
         res[name] = {"ms": ms, "bytes": B, "gbs": B / (ms * 1e-3) / 1e9, "wall_s": wall,
                      "layout_bytes": Bl, "layout_gbs": Bl / (ms * 1e-3) / 1e9, "shape": basis.shape(prec)}
-        print(f"[bench] {name}: {ms:.3f} ms/matvec  {B/1e9:.2f} GB algorithmic  "
+        log(f"[bench] {name}: {ms:.3f} ms/matvec  {B/1e9:.2f} GB algorithmic  "
               f"{res[name]['gbs']:.0f} GB/s ({100*res[name]['gbs']/PEAK_HBM_GBS:.1f}% of 8 TB/s)  "
               f"shape {res[name]['shape']}", flush=True)
 
     flux = None
     if not args.no_flux:
         flux = flux_scan(E, L, ctx, lanes=args.lanes)
-        print(f"[bench] flux scan {flux['solves']} solves: plain fp64 {flux['plain_fp64']['solves_per_s']}/s, "
+        log(f"[bench] flux scan {flux['solves']} solves: plain fp64 {flux['plain_fp64']['solves_per_s']}/s, "
               f"batched fp64 {flux['batched_fp64']['solves_per_s']}/s, batched fp32+refine "
               f"{flux['batched_fp32_refine']['solves_per_s']}/s (p99 {flux['batched_fp32_refine']['p99_ms']} ms)",
               flush=True)
     stream = None
     if not args.no_stream:
         stream = serving_stream(E, L, ctx, n_req=args.stream_requests, lanes=args.lanes)
-        print(f"[bench] serving stream {stream['requests']} requests dims {stream['dims']}: plain fp64 "
+        log(f"[bench] serving stream {stream['requests']} requests dims {stream['dims']}: plain fp64 "
               f"{stream['plain_fp64']['solves_per_s']}/s, server fp32+refine saturated "
               f"{stream['server_fp32_refine']['saturated_solves_per_s']}/s, Poisson stream p50/p99 "
               f"{stream['server_fp32_refine']['p50_ms']}/{stream['server_fp32_refine']['p99_ms']} ms "
