@@ -124,7 +124,8 @@ int sed_sector_hop_counts(sed_sector* sec, int64_t* hops_gated, int64_t* hops_in
  * host vectors of sector_dim ComplexF64; precision SED_PREC_FP64 or SED_PREC_FP32. */
 int sed_matvec(sed_sector* sec, const double* x, double* y, int precision);
 /* Same on device buffers of sector_dim complex elements (double2 for FP64, float2
- * for FP32) in the sector basis; ordered on the context's stream. */
+ * for FP32) in the sector basis, with or without twisted phases; ordered on the
+ * context's stream. */
 int sed_matvec_device(sed_sector* sec, const void* x_dev, void* y_dev, int precision);
 /* Time n_calls device matvecs on resident random vectors; returns the mean
  * kernel time in ms measured with HIP events on the context's stream. */

@@ -569,8 +569,18 @@ int sed_matvec_device(sed_sector* sec, const void* x_dev, void* y_dev, int preci
     hipStream_t s = sec->cat->ctx->stream;
     if (sec->has_gauge && precision == SED_PREC_FP64)
         return matvec_dev64(sec, (const double*)x_dev, (double*)y_dev, precision);
-    if (sec->has_gauge) return fail(SED_EUNSUPPORTED, "device FP32 matvec with twisted phases: use sed_matvec");
     Workspace& ws = *sec->cat->ctx->ws;
+    if (sec->has_gauge && precision == SED_PREC_FP32) {
+        // x' = D x (sector basis -> gauge-free basis), y = D^-1 H' x'
+        DevBuf<float> xp;
+        SED_TRY(xp.alloc((size_t)2 * sec->dim));
+        SED_TRY(launch_gauge32(sec, (const float*)x_dev, xp.p, true, s));
+        SED_TRY(grow(ws.xh32, (size_t)2 * sec->dim));
+        SED_TRY(launch_matvec32(sec, xp.p, (float*)y_dev, ws.xh32.p, s));
+        SED_TRY(launch_gauge32(sec, (const float*)y_dev, (float*)y_dev, false, s));
+        SED_HIP(hipStreamSynchronize(s));   // xp is released on return
+        return SED_OK;
+    }
     if (precision == SED_PREC_FP64) {
         SED_TRY(grow(ws.xh64, (size_t)2 * sec->dim));
         return launch_matvec64(sec, (const double*)x_dev, (double*)y_dev, ws.xh64.p, s);

@@ -195,6 +195,7 @@ int launch_matvec64(sed_sector* sec, const double* x, double* y, double* xhat, h
 int launch_matvec32(sed_sector* sec, const float* x, float* y, float* xhat, hipStream_t s);
 // gauge D: v <- D^{+-1} v  (forward = multiply by F, backward by conj F)
 int launch_gauge64(sed_sector* sec, double* v, bool forward, hipStream_t s);
+int launch_gauge32(sed_sector* sec, const float* v, float* out, bool forward, hipStream_t s);
 
 int solve_sector(sed_sector* sec, int nev, double tol, uint64_t seed, int maxiter, int precision,
                  std::vector<double>& evals, DevBuf<double2>* out_vecs, sed_solve_stats* stats, Workspace& ws);

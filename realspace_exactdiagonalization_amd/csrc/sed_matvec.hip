@@ -218,6 +218,27 @@ __global__ void __launch_bounds__(256) k_gauge(double2* __restrict__ v, const do
     }
 }
 
+// the same on FP32 vectors: out = F v (forward) or conj(F) v, F in FP64
+__global__ void __launch_bounds__(256) k_gauge32(const float2* v, float2* out,
+                                                 const double2* __restrict__ F, int64_t n, int forward) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        double2 f = F[r];
+        if (!forward) f.y = -f.y;
+        const double2 a = cmul(f, make_double2(v[r].x, v[r].y));
+        out[r] = make_float2((float)a.x, (float)a.y);
+    }
+}
+
+int launch_gauge32(sed_sector* sec, const float* v, float* out, bool forward, hipStream_t s) {
+    if (sec->dim == 0) return SED_OK;
+    const int64_t g = std::min<int64_t>((sec->dim + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(k_gauge32, dim3((unsigned)g), dim3(256), 0, s, reinterpret_cast<const float2*>(v),
+                       reinterpret_cast<float2*>(out), reinterpret_cast<const double2*>(sec->d_gauge.p), sec->dim,
+                       forward ? 1 : 0);
+    SED_HIP(hipGetLastError());
+    return SED_OK;
+}
+
 // F(rep_r) = prod_{i in rep_r} f(i)
 __global__ void __launch_bounds__(256) k_gauge_rows(const uint64_t* __restrict__ rep, const double2* __restrict__ f,
                                                     int n_site, int64_t dim, double2* __restrict__ F) {
