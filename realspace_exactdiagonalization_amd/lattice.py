@@ -46,11 +46,18 @@ class Lattice:
     """Periodic Bravais lattice with ``n_sub`` sublattices per cell."""
 
     def __init__(self, sample_size: Sequence[int], n_sub: int,
-                 pbc: Sequence[bool] = (True, True), name: str = "lattice"):
+                 pbc: Sequence[bool] = (True, True), name: str = "lattice",
+                 sub_crys: Optional[Sequence[Sequence[float]]] = None):
         self.sample_size = (int(sample_size[0]), int(sample_size[1]))
         self.n_sub = int(n_sub)
         self.pbc = (bool(pbc[0]), bool(pbc[1]))
         self.name = name
+        self.dim = 2
+        # crystal coordinates of each sublattice inside the cell (TightBinding's sub_crys_list)
+        sc = [(0.0, 0.0)] * self.n_sub if sub_crys is None else [(float(a), float(b)) for a, b in sub_crys]
+        if len(sc) != self.n_sub:
+            raise ValueError("sub_crys needs one offset per sublattice")
+        self.sub_crys: List[Tuple[float, float]] = sc
         L1, L2 = self.sample_size
         self.n_site = L1 * L2 * self.n_sub
         # site_list[i-1] = ((x, y), isub) with isub 1-based, like TightBinding
@@ -262,7 +269,7 @@ def haldane_honeycomb(L1: int, L2: int, t: float = 1.0, tp: float = 0.6, tpp: fl
                       n_filled: Optional[int] = None) -> Model:
     """Hard-core bosons on the Haldane honeycomb (``test/bosonic_fci.jl:22-112``,
     ``benchmark/benchmark.jl:114-146``)."""
-    lat = Lattice((L1, L2), 2, name="Haldane_Honeycomb")
+    lat = Lattice((L1, L2), 2, name="Haldane_Honeycomb", sub_crys=[(0.0, 0.0), (1 / 3, 1 / 3)])
     ph = 2.0 * math.pi * phi_over_2pi
     e = lambda s: complex(math.cos(s * ph), math.sin(s * ph))
     B = Bond
@@ -329,7 +336,7 @@ def checkerboard(L1: int, L2: int, t: float = 1.0, phi_over_2pi: float = 1 / 8,
                  V1: float = 1.0, V2: float = 0.0, V3: float = 0.0, lam: float = 1.0,
                  n_filled: Optional[int] = None, statistics: int = FERMIONIC) -> Model:
     """Spinless fermions on the checkerboard lattice (``phase_exploration/src/model.jl:1-71``)."""
-    lat = Lattice((L1, L2), 2, name="checkerboard")
+    lat = Lattice((L1, L2), 2, name="checkerboard", sub_crys=[(0.5, 0.0), (0.0, 0.5)])
     ph = 2.0 * math.pi * phi_over_2pi
     e = lambda s: complex(math.cos(s * ph), math.sin(s * ph))
     B = Bond

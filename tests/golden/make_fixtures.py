@@ -56,6 +56,31 @@ def main():
             "table_entries": int(re.search(r"projection table entries = (\d+)", cell43).group(1)),
         },
     }
+    # doc/charge_pump.ipynb: bosonic Haldane 2x3, 3 bosons, 17 fluxes 0:1/16:1 along x
+    # (cells 4, 5: flow and pump, polarization along y) and a 5x5 flux torus (cell 7)
+    nb = json.load(open(os.path.join(REF, "doc", "charge_pump.ipynb")))
+    pouts = {}
+    for i, c in enumerate(nb["cells"]):
+        pouts[i] = "".join("".join(o.get("text") or o.get("data", {}).get("text/plain") or "")
+                           for o in c.get("outputs", []))
+    num = r"(-?[0-9.]+(?:e-?[0-9]+)?)"
+    flow = re.search(r"Scanned (\d+) sectors at (\d+) flux points", pouts[4])
+    charges = re.search(r"Branch charges: \[" + num + ", " + num + r"\]", pouts[5])
+    final = re.findall(r"sector = \((\d+), (\d+)\), level = (\d+)", pouts[5])
+    data["charge_pump_notebook"] = {
+        "model": "haldane_boson_FCI 2x3 (test/bosonic_fci.jl params), filling 1/4",
+        "flow": {"n_sectors": int(flow.group(1)), "n_flux": int(flow.group(2))},
+        "pump": {
+            "pumped_charges": [float(charges.group(1)), float(charges.group(2))],
+            "min_gap": float(re.search(r"Minimum gap: " + num, pouts[5]).group(1)),
+            "selected_states_final": [[int(a), int(b), int(lv)] for a, b, lv in final],
+        },
+        "chern": {
+            "chern_number": float(re.search(r"Manifold Chern number: " + num, pouts[7]).group(1)),
+            "min_gap": float(re.search(r"Minimum gap: " + num, pouts[7]).group(1)),
+            "min_link_det": float(re.search(r"minimum link determinant: " + num, pouts[7]).group(1)),
+        },
+    }
     json.dump(data, open(OUT, "w"), indent=1)
     print("wrote", OUT)
 
