@@ -57,6 +57,8 @@ typedef struct sed_solve_stats {
     int64_t matvecs;          /* H|psi> applications */
     double residual_max;      /* max residual norm of the returned pairs */
     double seconds;           /* wall time of the solve */
+    int32_t full_orth_steps;  /* steps orthogonalised against the whole Krylov basis */
+    int32_t reserved;
 } sed_solve_stats;
 
 /* Thread-local message of the last failing call on this thread. */

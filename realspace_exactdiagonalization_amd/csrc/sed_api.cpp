@@ -87,7 +87,7 @@ extern "C" {
 
 const char* sed_last_error(void) { return g_last_error.c_str(); }
 
-int sed_version(void) { return 100; }
+int sed_version(void) { return 200; }
 
 int sed_create(int device, sed_context** out) {
     if (!out) return fail(SED_EINVAL, "out is NULL");

@@ -90,8 +90,9 @@ struct TileWalk {
     __device__ int tau(int it) const { return it % ntile; }
 };
 
-// part[i * G + b] = sum over block b's rows of conj(V_i[r]) w[r] for i <= j, and
-// part[(j+1) * G + b] = sum |w[r]|^2, j = *step.  A zero gate skips the pass.
+// part[i * G + b] = sum over block b's rows of conj(V_i[r]) w[r] for lo <= i <= j, and
+// part[(j+1) * G + b] = sum |w[r]|^2, j = *step, lo = *lo (pass_range).  w is
+// v_{j+1} itself when w_next.  A zero gate skips the pass.
 //
 // The block walks its rows in tiles of 64; each tile of 16 basis vectors is loaded with
 // one coalesced 64-row load per vector (lane = row) and staged in LDS.  Product on the
@@ -100,16 +101,22 @@ struct TileWalk {
 // tile), c = 0: sum vr wr + vi wi (real part), c = 1: sum vr wi - vi wr (imaginary part);
 // columns 2, 3 are zero.
 template <typename T>
-__global__ void __launch_bounds__(256) k_proj_mfma(const T* __restrict__ V, int64_t n, const int* __restrict__ step,
-                                                   const T* __restrict__ w, double2* __restrict__ part,
-                                                   const int* __restrict__ gate) {
+__global__ void __launch_bounds__(256) k_proj_mfma(const T* __restrict__ V0, int64_t n, const int* __restrict__ step,
+                                                   const T* __restrict__ w0, double2* __restrict__ part0,
+                                                   const int* __restrict__ gate, const int* __restrict__ lo,
+                                                   int w_next, int disp) {
     __shared__ double2 sv[16 * kTilePad];
     __shared__ double2 sw[kTileRows];
     __shared__ double red[256 / kWave][kProjTiles][kWave];
     if (gate && *gate == 0) return;
-    const int nv = *step + 1;               // basis vectors 0..nv-1, slot nv = w itself
-    const int ntile = (nv + 1 + 15) / 16;
+    PassRange pr;
+    if (!pass_range(step, lo, disp, pr)) return;
     const int G = gridDim.x;
+    const T* __restrict__ V = V0 + (int64_t)pr.lo * n;   // the range's first vector
+    const T* __restrict__ w = w_next ? V0 + (int64_t)(pr.j + 1) * n : w0;
+    double2* __restrict__ part = part0 + (int64_t)pr.lo * G;
+    const int nv = pr.nv;                   // range vectors 0..nv-1, slot nv = w itself
+    const int ntile = (nv + 1 + 15) / 16;
     const int64_t chunk = proj_chunk(n, G);
     const int64_t r0 = (int64_t)blockIdx.x * chunk;
     const TileWalk tw(r0, r0 + chunk < n ? r0 + chunk : n, ntile);
@@ -177,9 +184,11 @@ __global__ void __launch_bounds__(256) k_proj_mfma(const T* __restrict__ V, int6
     }
 }
 
-// out[r] = w[r] - sum_{i<=j} V_i[r] h[i] with out = w (to_next = 0) or V_{j+1}
-// (to_next = 1); each block also writes the partial |out|^2 of its rows (fixed grid).
-// A zero gate (second pass skipped) makes the update a copy.
+// out[r] = w[r] - sum_{lo<=i<=j} V_i[r] h[i] with out = w (to_next = 0) or V_{j+1}
+// (to_next = 1) and w = v_{j+1} itself when w_next (in place: every row is read before
+// the thread that owns it writes it); each block also writes the partial |out|^2 of its
+// rows (fixed grid).  A zero gate, or a range that the other kernel family takes, leaves
+// everything untouched.
 //
 // Tiles of 64 rows x 16 vectors are staged through LDS as in k_proj_mfma.  Product on
 // the 4x4x4 MFMA (4 blocks): D_b[i][c] over complex rows 16 wv + 4b + c of the tile,
@@ -188,18 +197,24 @@ __global__ void __launch_bounds__(256) k_proj_mfma(const T* __restrict__ V, int6
 // instruction its real part (A = (Re h, Im h)), the next its imaginary part
 // (A = (-Im h, Re h)).
 template <typename T>
-__global__ void __launch_bounds__(256) k_orth_update_mfma(T* V, int64_t n, const int* __restrict__ step,
-                                                          const double2* __restrict__ h, T* w, int to_next,
+__global__ void __launch_bounds__(256) k_orth_update_mfma(T* V0, int64_t n, const int* __restrict__ step,
+                                                          const double2* __restrict__ h, T* w0, int to_next,
                                                           double* __restrict__ normpart,
-                                                          const int* __restrict__ gate) {
+                                                          const int* __restrict__ gate, const int* __restrict__ lo,
+                                                          int w_next, int disp) {
     __shared__ double2 sv[16 * kTilePad];
     __shared__ double2 lh[kProjTiles * 16];
     __shared__ double red[256 / kWave];
-    const int nv = (gate && *gate == 0) ? 0 : *step + 1;
-    const int nout = *step + 1;
+    if (gate && *gate == 0) return;
+    PassRange pr;
+    if (!pass_range(step, lo, disp, pr)) return;
+    const int nv = pr.nv;
     const int ntile = (nv + 15) / 16;
-    for (int i = threadIdx.x; i < 16 * ntile; i += blockDim.x) lh[i] = i < nv ? h[i] : make_double2(0.0, 0.0);
-    T* out = to_next ? V + (int64_t)nout * n : w;
+    for (int i = threadIdx.x; i < 16 * ntile; i += blockDim.x)
+        lh[i] = i < nv ? h[pr.lo + i] : make_double2(0.0, 0.0);
+    const T* V = V0 + (int64_t)pr.lo * n;   // the range's first vector
+    T* out = to_next ? V0 + (int64_t)(pr.j + 1) * n : w0;
+    const T* w = w_next ? V0 + (int64_t)(pr.j + 1) * n : w0;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int k = lane >> 4, b = (lane >> 2) & 3, q = lane & 3;
     const int G = gridDim.x;
@@ -207,19 +222,14 @@ __global__ void __launch_bounds__(256) k_orth_update_mfma(T* V, int64_t n, const
     const int64_t r0 = (int64_t)blockIdx.x * chunk;
     const int64_t r1 = r0 + chunk < n ? r0 + chunk : n;
     double nrm = 0.0;
-    if (ntile == 0) {   // copy w (second pass skipped)
-        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-            const T o = w[r];
-            out[r] = o;
-            const double2 od = to_d2(o);
-            nrm += od.x * od.x + od.y * od.y;
-        }
-    } else {
+    {
         const TileWalk tw(r0, r1, ntile);
         double acc = 0.0;
-        // tile loads plus this wave's 16 rows of w for the row tile's final subtraction
+        // tile loads plus this wave's 16 rows of w for the row tile's final subtraction;
+        // slots without a vector re-read the range's first vector (never w, which may be
+        // the output)
         auto load = [&](int it, T (&x)[4], int& keep, T& xe) {
-            load_vec_tile(V, w, n, tw.t0(it), r1, tw.tau(it), nv, 0, wv, lane, x, keep);
+            load_vec_tile(V, V, n, tw.t0(it), r1, tw.tau(it), nv, 0, wv, lane, x, keep);
             const int64_t re = tw.t0(it) + 16 * wv + (lane & 15);
             xe = w[re < r1 ? re : r1 - 1];
         };

@@ -5,9 +5,13 @@
 // (Lanczos with restarts and full re-orthogonalisation) and solves sectors of
 // dimension <= 500 densely.  Here:
 //   * thick-restart Lanczos (Wu & Simon) with a basis of at most m vectors resident in
-//     HBM; every new vector is orthogonalised twice against the whole basis
-//     (classical Gram-Schmidt, "twice is enough"), and the alpha/beta recurrence and
-//     every projection coefficient are carried in FP64 whatever the vector precision;
+//     HBM and partial re-orthogonalisation (Simon): each new vector is orthogonalised
+//     against the two previous ones, and against the whole basis (classical
+//     Gram-Schmidt, a second pass when the first cancelled most of it) only when
+//     Simon's recurrence estimates a loss of orthogonality beyond sqrt(eps), for the
+//     two steps after a restart, and once more after each such step; the alpha/beta
+//     recurrence and every projection coefficient are carried in FP64 whatever the
+//     vector precision;
 //   * one Lanczos step is a fixed sequence of kernels whose step index j lives in device
 //     memory: coefficients, norms, alpha_j and beta_j never leave the device inside a
 //     step, so steps run back to back without host synchronisation and the sequence is
@@ -44,15 +48,22 @@ constexpr int kUpdGrid = 1024;  // fixed grid of the update / norm kernels
 // w = H v_j, then two classical Gram-Schmidt passes against V[0..j], then
 // v_{j+1} = w' / |w'| and its xhat.
 
-// part[i * G + b] = sum over block b's rows of conj(V_i[r]) w[r] for i <= j, and
-// part[(j+1) * G + b] = sum |w[r]|^2.
+// part[i * G + b] = sum over block b's rows of conj(V_i[r]) w[r] for lo <= i <= j, and
+// part[(j+1) * G + b] = sum |w[r]|^2 (range and dispatch as pass_range; w = v_{j+1}
+// when w_next).
 template <typename T>
-__global__ void __launch_bounds__(kRedBlock) k_dots(const T* __restrict__ V, int64_t n, const int* __restrict__ step,
-                                                    const T* __restrict__ w, double2* __restrict__ part,
-                                                    const int* __restrict__ gate) {
+__global__ void __launch_bounds__(kRedBlock) k_dots(const T* __restrict__ V0, int64_t n, const int* __restrict__ step,
+                                                    const T* __restrict__ w0, double2* __restrict__ part0,
+                                                    const int* __restrict__ gate, const int* __restrict__ lo,
+                                                    int w_next, int disp) {
     __shared__ double2 red[kDotGroup][kRedBlock / kWave];
     if (gate && *gate == 0) return;   // second pass not needed this step
-    const int nv = *step + 1;
+    PassRange pr;
+    if (!pass_range(step, lo, disp, pr)) return;
+    const T* __restrict__ V = V0 + (int64_t)pr.lo * n;
+    const T* __restrict__ w = w_next ? V0 + (int64_t)(pr.j + 1) * n : w0;
+    double2* __restrict__ part = part0 + (int64_t)pr.lo * gridDim.x;
+    const int nv = pr.nv;
     const int total = nv + 1;
     const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t r0 = (int64_t)blockIdx.x * chunk;
@@ -94,14 +105,15 @@ __global__ void __launch_bounds__(kRedBlock) k_dots(const T* __restrict__ V, int
     }
 }
 
-// out[i] = sum_b part[i * G + b] for i <= j + 1 (fixed order); launched with m + 2 blocks.
+// out[i] = sum_b part[i * G + b] for lo <= i <= j + 1 (fixed order); launched with
+// m + 2 blocks.
 __global__ void __launch_bounds__(kRedBlock) k_reduce(const double2* __restrict__ part, int G,
                                                       const int* __restrict__ step, double2* __restrict__ out,
-                                                      const int* __restrict__ gate) {
+                                                      const int* __restrict__ gate, const int* __restrict__ lo) {
     __shared__ double2 red[kRedBlock / kWave];
     if (gate && *gate == 0) return;
     const int i = blockIdx.x;
-    if (i > *step + 1) return;
+    if (i > *step + 1 || i < (lo ? *lo : 0)) return;
     double2 acc = make_double2(0.0, 0.0);
     for (int b = threadIdx.x; b < G; b += kRedBlock) {
         acc.x += part[(int64_t)i * G + b].x;
@@ -120,20 +132,26 @@ __global__ void __launch_bounds__(kRedBlock) k_reduce(const double2* __restrict_
     }
 }
 
-// out[r] = w[r] - sum_{i<=j} V_i[r] h[i]; out = w (to_next = 0) or V_{j+1} (to_next = 1).
-// Each block also writes the partial |out|^2 of its rows (fixed grid).  A zero gate
-// (second pass skipped) makes the update a copy.
+// out[r] = w[r] - sum_{lo<=i<=j} V_i[r] h[i]; out = w (to_next = 0) or V_{j+1}
+// (to_next = 1), w = v_{j+1} itself when w_next (in place, row by row).  Each block also
+// writes the partial |out|^2 of its rows (fixed grid).  A zero gate, or a range that the
+// other kernel family takes, leaves everything untouched.
 template <typename T>
-__global__ void __launch_bounds__(256) k_update(T* V, int64_t n, const int* __restrict__ step,
-                                                const double2* __restrict__ h, T* w, int to_next,
-                                                double* __restrict__ normpart, const int* __restrict__ gate) {
+__global__ void __launch_bounds__(256) k_update(T* V0, int64_t n, const int* __restrict__ step,
+                                                const double2* __restrict__ h, T* w0, int to_next,
+                                                double* __restrict__ normpart, const int* __restrict__ gate,
+                                                const int* __restrict__ lo, int w_next, int disp) {
     __shared__ double2 lh[kMaxBasis + 2];
     __shared__ double red[256 / kWave];
-    const int nv = (gate && *gate == 0) ? 0 : *step + 1;
-    const int nout = *step + 1;
-    for (int i = threadIdx.x; i < nv; i += blockDim.x) lh[i] = h[i];
+    if (gate && *gate == 0) return;
+    PassRange pr;
+    if (!pass_range(step, lo, disp, pr)) return;
+    const int nv = pr.nv;
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) lh[i] = h[pr.lo + i];
     __syncthreads();
-    T* out = to_next ? V + (int64_t)nout * n : w;
+    const T* V = V0 + (int64_t)pr.lo * n;
+    T* out = to_next ? V0 + (int64_t)(pr.j + 1) * n : w0;
+    const T* w = w_next ? V0 + (int64_t)(pr.j + 1) * n : w0;
     double nrm = 0.0;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
         double2 a = to_d2(w[r]);
@@ -158,12 +176,46 @@ __global__ void __launch_bounds__(256) k_update(T* V, int64_t n, const int* __re
     }
 }
 
-// Single block, after the first pass: |w'|^2 from the update's partials against |w|^2
-// (h1[j+1]); gate = 1 when the second pass must run, else h2[j] = 0 for k_finalize.
+// Partial re-orthogonalisation state (device memory, next to the step counter).
+// omega[r % 3][k] estimates |v_r^H v_k| for the rows r = j-1, j, j+1 around step j
+// (Simon's recurrence); T is the projected matrix: alpha[k] on the diagonal (the kept
+// Ritz values after a thick restart), beta[k] = T[k+1][k] for k >= kk, and the
+// restart's arrowhead T[kk][k] = arrow[k] for k < kk.
+struct ProState {
+    int lo;        // first basis vector of the current step's first pass
+    int force;     // steps, from the current one, that run the second pass in any case
+    int kk;        // kept Ritz vectors of this cycle (the arrowhead row)
+    int nfull;     // steps with a pass over the whole basis
+    int enabled;   // 0: every pass spans the whole basis
+    int pad[3];
+    double eps;     // unit roundoff of the stored vectors
+    double delta;   // semi-orthogonality threshold, sqrt(eps)
+    double anorm;   // running estimate of |T|
+    double pad2;
+    double arrow[kMaxBasis];
+    double omega[3][kMaxBasis + 2];
+};
+
+// Single block, after the first pass: |w'|^2 from the update's partials.  The second
+// pass (over the whole basis) runs when the first one cancelled most of w
+// (|w'|^2 <= eta^2 |w|^2, Daniel-Gragg-Kaufman-Stewart), when the step is forced, or
+// when the first pass was local (lo > 0) and the recurrence
+//   beta_j omega_{j+1,k} = (T omega_j)_k - beta_{j-1} omega_{j-1,k} - alpha_j omega_{j,k}
+//                          +- eps |T|   (rounding of the step)
+// puts some |omega_{j+1,k}|, k <= j-2, above sqrt(eps); the step after such a step is
+// forced too (v_j itself is only semi-orthogonal, and its large coefficient in the next
+// vector carries that into it), as are the two steps after a thick restart.  A forced
+// step keeps its local first pass, which removes the large components, so that the
+// second pass over the semi-orthogonal basis leaves eps-level overlaps.  After a
+// second pass (or a first pass over an orthonormal basis, steps 0 and 1)
+// omega_{j+1,.} is reset to eps; the two vectors a local pass orthogonalises against
+// get eps.
 __global__ void __launch_bounds__(kRedBlock) k_reorth_check(const int* __restrict__ step,
                                                             const double* __restrict__ normpart, int G,
                                                             const double2* __restrict__ h1, double2* __restrict__ h2,
-                                                            int* __restrict__ gate) {
+                                                            int* __restrict__ gate, ProState* __restrict__ pro,
+                                                            const double* __restrict__ alpha,
+                                                            const double* __restrict__ beta) {
     __shared__ double red[kRedBlock / kWave];
     double acc = 0.0;
     for (int b = threadIdx.x; b < G; b += kRedBlock) acc += normpart[b];
@@ -174,8 +226,55 @@ __global__ void __launch_bounds__(kRedBlock) k_reorth_check(const int* __restric
     double nrm2 = 0.0;
     for (int k = 0; k < kRedBlock / kWave; ++k) nrm2 += red[k];
     const int j = *step;
-    const int need = nrm2 > kReorthEta2 * h1[j + 1].x ? 0 : 1;
-    *gate = need;
+    bool need = !(nrm2 > kReorthEta2 * h1[j + 1].x);
+    if (pro->force > 0) {
+        need = true;
+        --pro->force;
+    }
+    const int lo = pro->lo;
+    double* wn = pro->omega[(j + 1) % 3];
+    const double eps = pro->eps;
+    if (pro->enabled && lo > 0 && !need) {
+        const double* wj = pro->omega[j % 3];
+        const double* wm = pro->omega[(j + 2) % 3];   // row j - 1
+        const int kk = pro->kk;
+        const double bj = sqrt(nrm2);
+        const double aj = h1[j].x;
+        const double bjm = beta[j - 1];
+        const double noise = eps * fmax(pro->anorm, fabs(aj) + bj + bjm);
+        double mx = 0.0;
+        for (int k = 0; k <= j - 2; ++k) {
+            double tw = alpha[k] * wj[k];
+            if (k < kk) {
+                tw += pro->arrow[k] * wj[kk];
+            } else {
+                tw += beta[k] * wj[k + 1];
+                if (k > kk) {
+                    tw += beta[k - 1] * wj[k - 1];
+                } else {
+                    for (int b = 0; b < kk; ++b) tw += pro->arrow[b] * wj[b];
+                }
+            }
+            double x = tw - bjm * wm[k] - aj * wj[k];
+            x += x < 0.0 ? -noise : noise;
+            x /= bj;
+            wn[k] = x;
+            mx = fmax(mx, fabs(x));
+        }
+        if (!(mx <= pro->delta)) {   // NaN (bj = 0) included
+            need = true;
+            pro->force = pro->force > 1 ? pro->force : 1;
+        } else {
+            wn[j - 1] = eps;
+            wn[j] = eps;
+        }
+    }
+    if (need || lo == 0) {
+        for (int k = 0; k <= j; ++k) wn[k] = eps;
+        ++pro->nfull;
+    }
+    wn[j + 1] = 1.0;
+    *gate = need ? 1 : 0;
     if (!need) h2[j] = make_double2(0.0, 0.0);
 }
 
@@ -186,7 +285,8 @@ __global__ void __launch_bounds__(kRedBlock) k_finalize(int* step, const double2
                                                         const double2* __restrict__ h2,
                                                         const double* __restrict__ normpart, int G,
                                                         double* __restrict__ alpha, double* __restrict__ beta,
-                                                        double* __restrict__ scale, int* __restrict__ flag) {
+                                                        double* __restrict__ scale, int* __restrict__ flag,
+                                                        ProState* __restrict__ pro) {
     __shared__ double red[kRedBlock / kWave];
     double acc = 0.0;
     for (int b = threadIdx.x; b < G; b += kRedBlock) acc += normpart[b];
@@ -206,6 +306,9 @@ __global__ void __launch_bounds__(kRedBlock) k_finalize(int* step, const double2
     *scale = brk ? 0.0 : 1.0 / b;
     if (brk && *flag == 0) *flag = j + 1;
     *step = j + 1;
+    // the next step's first pass: vectors j, j + 1 only, or the whole basis
+    pro->anorm = fmax(pro->anorm, fabs(a) + b + (j > 0 ? beta[j - 1] : 0.0));
+    pro->lo = pro->enabled ? j : 0;
 }
 
 // v_j *= scale and xhat = sqrt(stab) v_j for j = *step (the vector the next step applies H to).
@@ -389,6 +492,7 @@ struct SolveOut {
     int converged = 0;
     int iters = 0;
     int restarts = 0;
+    int full_steps = 0;   // steps orthogonalised against the whole basis
     int64_t matvecs = 0;
 };
 
@@ -399,18 +503,25 @@ static bool reorth_mfma() {
     return !(e && std::strcmp(e, "valu") == 0);
 }
 
+// Partial re-orthogonalisation (default), or a pass over the whole basis at every step
+// with SED_PRO=0 (kept for comparison; read once per solve).
+static bool pro_enabled() {
+    const char* e = std::getenv("SED_PRO");
+    return !(e && e[0] == '0');
+}
+
 static bool graphs_enabled() {
     const char* e = std::getenv("SED_LANCZOS_GRAPH");
     return !(e && e[0] == '0');
 }
 
 // Device state block (doubles): h1 (2*(m+2)), h2 (2*(m+2)), alpha (m), beta (m), scale (1);
-// ints after it: step, flag.
+// ints after it: step, flag, gate; then the partial re-orthogonalisation state.
 template <typename T>
 class ThickLanczos {
    public:
     ThickLanczos(sed_sector* sec, int m, Workspace& ws)
-        : sec_(sec), n_(sec->dim), m_(m), ws_(ws), st_(ws.stream), mfma_(reorth_mfma()) {}
+        : sec_(sec), n_(sec->dim), m_(m), ws_(ws), st_(ws.stream), mfma_(reorth_mfma()), pro_on_(pro_enabled()) {}
 
     int init() {
         SED_TRY(VecTraits<T>::reserve(ws_, n_, m_));
@@ -418,7 +529,8 @@ class ThickLanczos {
         SED_TRY(grow(ws_.part, (size_t)2 * (m_ + 2) * G_ + kUpdGrid));
         SED_TRY(grow(ws_.S, (size_t)(m_ + 1) * (m_ + 1)));
         const size_t nd = 4 * (size_t)(m_ + 2) + 2 * (size_t)m_ + 1;
-        SED_TRY(grow(ws_.state, nd + 4));
+        const size_t npro = (sizeof(ProState) + sizeof(double) - 1) / sizeof(double);
+        SED_TRY(grow(ws_.state, nd + 4 + npro));
         SED_TRY(ws_.ensure_host(nd + 8));
         V_ = reinterpret_cast<T*>(VecTraits<T>::V(ws_));
         w_ = reinterpret_cast<T*>(VecTraits<T>::W(ws_));
@@ -434,9 +546,22 @@ class ThickLanczos {
         step_ = reinterpret_cast<int*>(scale_ + 1);
         flag_ = step_ + 1;
         gate_ = step_ + 2;
+        pro_ = reinterpret_cast<ProState*>(st + nd + 4);
         int zero[2] = {0, 0};
         SED_HIP(hipMemcpyAsync(step_, zero, sizeof(zero), hipMemcpyHostToDevice, st_));
-        SED_HIP(hipStreamSynchronize(st_));  // `zero` is a stack temporary
+        ProState p{};
+        p.enabled = pro_on_ ? 1 : 0;
+        p.eps = VecTraits<T>::fp64 ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
+        p.delta = std::sqrt(p.eps);
+        SED_HIP(hipMemcpyAsync(pro_, &p, sizeof(p), hipMemcpyHostToDevice, st_));
+        SED_HIP(hipStreamSynchronize(st_));  // `zero` and `p` are stack temporaries
+        return SED_OK;
+    }
+    // steps of the solve whose orthogonalisation spanned the whole basis
+    int read_full_steps(int& out) {
+        SED_HIP(hipMemcpyAsync(&ws_.hcoef[0], &pro_->nfull, sizeof(int), hipMemcpyDeviceToHost, st_));
+        SED_HIP(hipStreamSynchronize(st_));
+        std::memcpy(&out, &ws_.hcoef[0], sizeof(int));
         return SED_OK;
     }
     T* vec(int i) { return V_ + (size_t)i * n_; }
@@ -511,14 +636,17 @@ class ThickLanczos {
                 for (int i = 0; i < kk; ++i) Sk[(size_t)jj * kk + i] = Sm[(size_t)jj * jm + i];
             SED_TRY(ritz(jm, Sk, kk));
             SED_HIP(hipMemcpyAsync(vec(kk), vec(jm), (size_t)n_ * sizeof(T), hipMemcpyDeviceToDevice, st_));
-            SED_TRY(set_step_and_xhat(kk));
             std::fill(T_.begin(), T_.end(), 0.0);
+            std::vector<double> arrow(kk);
             for (int i = 0; i < kk; ++i) {
                 T_[(size_t)i * m_ + i] = theta[i];
                 const double sc = beta * Sm[(size_t)(jm - 1) * jm + i];
                 T_[(size_t)kk * m_ + i] = sc;
                 T_[(size_t)i * m_ + kk] = sc;
+                arrow[i] = sc;
             }
+            SED_TRY(pro_restart(kk, theta, arrow));
+            SED_TRY(set_step_and_xhat(kk));
             k = kk;
             ++out.restarts;
         }
@@ -535,7 +663,7 @@ class ThickLanczos {
             for (int i = 0; i < nk; ++i) Sk[(size_t)jj * nk + i] = Sm[(size_t)jj * jm + i];
         SED_TRY(ritz(jm, Sk, nk));
         nritz_ = nk;
-        SED_HIP(hipStreamSynchronize(st_));
+        SED_TRY(read_full_steps(out.full_steps));
         return SED_OK;
     }
 
@@ -548,37 +676,38 @@ class ThickLanczos {
     // one Lanczos step as plain launches
     int launch_step() {
         SED_TRY(launch_matvec_xhat(sec_, xh_, w_, VecTraits<T>::fp64, st_));
-        // first classical Gram-Schmidt pass; the second only when the first cancelled most
-        // of w (decided on the device)
-        SED_TRY(launch_orth_pass(h1_, 0, nullptr));
+        // first classical Gram-Schmidt pass over v_lo..v_j (lo decided on the device) into
+        // v_{j+1}; a second pass over the whole basis, in place, only when the device
+        // decides so (partial re-orthogonalisation or cancellation)
+        SED_TRY(launch_orth_pass(h1_, &pro_->lo, 0, nullptr));
         hipLaunchKernelGGL(k_reorth_check, dim3(1), dim3(kRedBlock), 0, st_, step_, normpart_, kUpdGrid, h1_, h2_,
-                           gate_);
-        SED_TRY(launch_orth_pass(h2_, 1, gate_));
+                           gate_, pro_, alpha_, beta_);
+        SED_TRY(launch_orth_pass(h2_, nullptr, 1, gate_));
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kRedBlock), 0, st_, step_, h1_, h2_, normpart_, kUpdGrid, alpha_,
-                           beta_, scale_, flag_);
+                           beta_, scale_, flag_, pro_);
         hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n_)), dim3(256), 0, st_, V_, (int64_t)n_, step_, scale_,
                            sec_->d_stab.p, xh_);
         SED_HIP(hipGetLastError());
         return SED_OK;
     }
 
-    // One Gram-Schmidt pass: h = V[0..j]^H w (and |w|^2), then w - V h into w
-    // (to_next = 0) or into v_{j+1} (to_next = 1), with the norm partials of the result.
-    int launch_orth_pass(double2* h, int to_next, const int* gate) {
-        if (mfma_) {
-            hipLaunchKernelGGL(k_proj_mfma<T>, dim3(G_), dim3(256), 0, st_, V_, (int64_t)n_, step_, w_, part_,
-                               gate);
-        } else {
-            hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_, gate);
-        }
-        hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h, gate);
-        if (mfma_) {
+    // One Gram-Schmidt pass: h = V[lo..j]^H w (and |w|^2), then w - V h into v_{j+1},
+    // lo = *lo (0 without a pointer), w = the matvec output (w_next = 0) or v_{j+1}
+    // itself (w_next = 1, in place).  With the matrix-core kernels both kernel families
+    // are launched and the range size picks the one that works (pass_range).
+    int launch_orth_pass(double2* h, const int* lo, int w_next, const int* gate) {
+        const int dm = mfma_ ? 1 : 0, dv = mfma_ ? 2 : 0;
+        if (mfma_)
+            hipLaunchKernelGGL(k_proj_mfma<T>, dim3(G_), dim3(256), 0, st_, V_, (int64_t)n_, step_, w_, part_, gate,
+                               lo, w_next, dm);
+        hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_, gate, lo,
+                           w_next, dv);
+        hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h, gate, lo);
+        if (mfma_)
             hipLaunchKernelGGL(k_orth_update_mfma<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h, w_,
-                               to_next, normpart_, gate);
-        } else {
-            hipLaunchKernelGGL(k_update<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h, w_, to_next,
-                               normpart_, gate);
-        }
+                               1, normpart_, gate, lo, w_next, dm);
+        hipLaunchKernelGGL(k_update<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h, w_, 1, normpart_,
+                           gate, lo, w_next, dv);
         SED_HIP(hipGetLastError());
         return SED_OK;
     }
@@ -632,6 +761,19 @@ class ThickLanczos {
         SED_HIP(hipGetLastError());
         return SED_OK;
     }
+    // after a thick restart to kk kept Ritz vectors: their Ritz values on the diagonal of
+    // T, no coupling among them, the arrowhead T[kk][i]; step kk's first pass spans the
+    // whole basis (the arrowhead), and steps kk and kk + 1 run the second pass.
+    int pro_restart(int kk, const std::vector<double>& theta, const std::vector<double>& arrow) {
+        std::vector<double> zeros((size_t)kk, 0.0);
+        const int hdr[3] = {0, pro_on_ ? 2 : 0, kk};   // lo, force, kk
+        SED_HIP(hipMemcpyAsync(alpha_, theta.data(), (size_t)kk * sizeof(double), hipMemcpyHostToDevice, st_));
+        SED_HIP(hipMemcpyAsync(beta_, zeros.data(), (size_t)kk * sizeof(double), hipMemcpyHostToDevice, st_));
+        SED_HIP(hipMemcpyAsync(pro_->arrow, arrow.data(), (size_t)kk * sizeof(double), hipMemcpyHostToDevice, st_));
+        SED_HIP(hipMemcpyAsync(&pro_->lo, hdr, sizeof(hdr), hipMemcpyHostToDevice, st_));
+        SED_HIP(hipStreamSynchronize(st_));  // host temporaries
+        return SED_OK;
+    }
     // after a restart: step = j and xhat from v_j (v_j already unit norm)
     int set_step_and_xhat(int j) {
         const double one = 1.0;
@@ -680,7 +822,9 @@ class ThickLanczos {
     double2 *h1_ = nullptr, *h2_ = nullptr;
     double *alpha_ = nullptr, *beta_ = nullptr, *scale_ = nullptr;
     int *step_ = nullptr, *flag_ = nullptr, *gate_ = nullptr;
+    ProState* pro_ = nullptr;
     bool mfma_ = true;
+    bool pro_on_ = true;
 };
 
 int64_t solve_footprint(const sed_sector* sec, int nev, int precision) {
@@ -773,6 +917,7 @@ int solve_sector(sed_sector* sec, int nev, double tol, uint64_t seed, int maxite
         st.iters_fp32 = o.iters;
         st.matvecs += o.matvecs;
         st.restarts += o.restarts;
+        st.full_orth_steps += o.full_steps;
         if (precision == SED_PREC_FP32) {
             evals = o.evals;
             st.converged = o.converged;
@@ -821,6 +966,7 @@ int solve_sector(sed_sector* sec, int nev, double tol, uint64_t seed, int maxite
     }
     st.matvecs += o.matvecs;
     st.restarts += o.restarts;
+    st.full_orth_steps += o.full_steps;
     evals = o.evals;
     st.converged = o.converged;
     fill_resid(st, o);

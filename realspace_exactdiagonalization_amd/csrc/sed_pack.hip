@@ -470,6 +470,7 @@ class PackEngine {
         job->stats.iters_fp32 = q.iters[0];
         job->stats.iters_fp64 = q.iters[1];
         job->stats.matvecs = q.iters[0] + q.iters[1];
+        job->stats.full_orth_steps = q.iters[0] + q.iters[1];   // every packed step spans the basis
         job->stats.restarts = q.restarts;
         job->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - q.t0).count();
         if (q.off >= 0) arena_.release(q.off, q.bytes);

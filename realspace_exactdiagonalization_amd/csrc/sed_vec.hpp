@@ -15,6 +15,25 @@ constexpr int kMaxBasis = 64;   // largest Krylov basis (m)
 // 1/eta of orthogonality relative to the vector precision.
 constexpr double kReorthEta2 = 0.01;
 
+// Basis vectors of one Gram-Schmidt pass of step j = *step: lo..j with lo = *lo (0
+// without a range pointer).  Two kernel families are launched for each pass and one of
+// them works: disp 1 (matrix-core kernels) runs when the range holds at least
+// kMfmaMinVectors vectors, disp 2 (row-wise loops) when it holds fewer, disp 0 always.
+// A one-tile MFMA pass costs a fixed latency-bound time that a row-wise pass over a few
+// vectors undercuts.
+constexpr int kMfmaMinVectors = 5;
+struct PassRange {
+    int j, lo, nv;
+};
+__device__ __forceinline__ bool pass_range(const int* step, const int* lo, int disp, PassRange& r) {
+    r.j = *step;
+    r.lo = lo ? *lo : 0;
+    r.nv = r.j + 1 - r.lo;
+    if (disp == 1) return r.nv >= kMfmaMinVectors;
+    if (disp == 2) return r.nv < kMfmaMinVectors;
+    return true;
+}
+
 // Krylov basis size for a sector of dimension n and nev wanted eigenpairs.
 inline int krylov_dim(int64_t n, int nev) {
     int m = std::max(2 * nev + 24, 40);
