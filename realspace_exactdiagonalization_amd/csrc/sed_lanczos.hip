@@ -176,40 +176,8 @@ __global__ void __launch_bounds__(256) k_update(T* V0, int64_t n, const int* __r
     }
 }
 
-// Partial re-orthogonalisation state (device memory, next to the step counter).
-// omega[r % 3][k] estimates |v_r^H v_k| for the rows r = j-1, j, j+1 around step j
-// (Simon's recurrence); T is the projected matrix: alpha[k] on the diagonal (the kept
-// Ritz values after a thick restart), beta[k] = T[k+1][k] for k >= kk, and the
-// restart's arrowhead T[kk][k] = arrow[k] for k < kk.
-struct ProState {
-    int lo;        // first basis vector of the current step's first pass
-    int force;     // steps, from the current one, that run the second pass in any case
-    int kk;        // kept Ritz vectors of this cycle (the arrowhead row)
-    int nfull;     // steps with a pass over the whole basis
-    int enabled;   // 0: every pass spans the whole basis
-    int pad[3];
-    double eps;     // unit roundoff of the stored vectors
-    double delta;   // semi-orthogonality threshold, sqrt(eps)
-    double anorm;   // running estimate of |T|
-    double pad2;
-    double arrow[kMaxBasis];
-    double omega[3][kMaxBasis + 2];
-};
-
-// Single block, after the first pass: |w'|^2 from the update's partials.  The second
-// pass (over the whole basis) runs when the first one cancelled most of w
-// (|w'|^2 <= eta^2 |w|^2, Daniel-Gragg-Kaufman-Stewart), when the step is forced, or
-// when the first pass was local (lo > 0) and the recurrence
-//   beta_j omega_{j+1,k} = (T omega_j)_k - beta_{j-1} omega_{j-1,k} - alpha_j omega_{j,k}
-//                          +- eps |T|   (rounding of the step)
-// puts some |omega_{j+1,k}|, k <= j-2, above sqrt(eps); the step after such a step is
-// forced too (v_j itself is only semi-orthogonal, and its large coefficient in the next
-// vector carries that into it), as are the two steps after a thick restart.  A forced
-// step keeps its local first pass, which removes the large components, so that the
-// second pass over the semi-orthogonal basis leaves eps-level overlaps.  After a
-// second pass (or a first pass over an orthonormal basis, steps 0 and 1)
-// omega_{j+1,.} is reset to eps; the two vectors a local pass orthogonalises against
-// get eps.
+// Single block, after the first pass: |w'|^2 from the update's partials; pro_decide
+// sets the gate of the second pass (and clears h2[j] when it does not run).
 __global__ void __launch_bounds__(kRedBlock) k_reorth_check(const int* __restrict__ step,
                                                             const double* __restrict__ normpart, int G,
                                                             const double2* __restrict__ h1, double2* __restrict__ h2,
@@ -226,54 +194,7 @@ __global__ void __launch_bounds__(kRedBlock) k_reorth_check(const int* __restric
     double nrm2 = 0.0;
     for (int k = 0; k < kRedBlock / kWave; ++k) nrm2 += red[k];
     const int j = *step;
-    bool need = !(nrm2 > kReorthEta2 * h1[j + 1].x);
-    if (pro->force > 0) {
-        need = true;
-        --pro->force;
-    }
-    const int lo = pro->lo;
-    double* wn = pro->omega[(j + 1) % 3];
-    const double eps = pro->eps;
-    if (pro->enabled && lo > 0 && !need) {
-        const double* wj = pro->omega[j % 3];
-        const double* wm = pro->omega[(j + 2) % 3];   // row j - 1
-        const int kk = pro->kk;
-        const double bj = sqrt(nrm2);
-        const double aj = h1[j].x;
-        const double bjm = beta[j - 1];
-        const double noise = eps * fmax(pro->anorm, fabs(aj) + bj + bjm);
-        double mx = 0.0;
-        for (int k = 0; k <= j - 2; ++k) {
-            double tw = alpha[k] * wj[k];
-            if (k < kk) {
-                tw += pro->arrow[k] * wj[kk];
-            } else {
-                tw += beta[k] * wj[k + 1];
-                if (k > kk) {
-                    tw += beta[k - 1] * wj[k - 1];
-                } else {
-                    for (int b = 0; b < kk; ++b) tw += pro->arrow[b] * wj[b];
-                }
-            }
-            double x = tw - bjm * wm[k] - aj * wj[k];
-            x += x < 0.0 ? -noise : noise;
-            x /= bj;
-            wn[k] = x;
-            mx = fmax(mx, fabs(x));
-        }
-        if (!(mx <= pro->delta)) {   // NaN (bj = 0) included
-            need = true;
-            pro->force = pro->force > 1 ? pro->force : 1;
-        } else {
-            wn[j - 1] = eps;
-            wn[j] = eps;
-        }
-    }
-    if (need || lo == 0) {
-        for (int k = 0; k <= j; ++k) wn[k] = eps;
-        ++pro->nfull;
-    }
-    wn[j + 1] = 1.0;
+    const bool need = pro_decide(pro, j, nrm2, h1[j + 1].x, h1[j].x, alpha, beta);
     *gate = need ? 1 : 0;
     if (!need) h2[j] = make_double2(0.0, 0.0);
 }
@@ -306,9 +227,7 @@ __global__ void __launch_bounds__(kRedBlock) k_finalize(int* step, const double2
     *scale = brk ? 0.0 : 1.0 / b;
     if (brk && *flag == 0) *flag = j + 1;
     *step = j + 1;
-    // the next step's first pass: vectors j, j + 1 only, or the whole basis
-    pro->anorm = fmax(pro->anorm, fabs(a) + b + (j > 0 ? beta[j - 1] : 0.0));
-    pro->lo = pro->enabled ? j : 0;
+    pro_advance(pro, j, a, b, beta);
 }
 
 // v_j *= scale and xhat = sqrt(stab) v_j for j = *step (the vector the next step applies H to).
@@ -496,11 +415,16 @@ struct SolveOut {
     int64_t matvecs = 0;
 };
 
-// Re-orthogonalisation and restart rotation on the FP64 matrix cores (default), or as
-// row-wise VALU loops with SED_REORTH=valu (kept for comparison; read once per solve).
-static bool reorth_mfma() {
+// Re-orthogonalisation and restart rotation on the FP64 matrix cores for sectors of at
+// least kMfmaMinRows rows, as row-wise VALU loops below that: there the basis sits in the
+// caches and a step is bound by its launch count, which a single kernel family keeps
+// lowest.  SED_REORTH=valu / SED_REORTH=mfma force either form (read once per solve).
+constexpr int64_t kMfmaMinRows = 1 << 17;
+static bool reorth_mfma(int64_t n) {
     const char* e = std::getenv("SED_REORTH");
-    return !(e && std::strcmp(e, "valu") == 0);
+    if (e && std::strcmp(e, "valu") == 0) return false;
+    if (e && std::strcmp(e, "mfma") == 0) return true;
+    return n >= kMfmaMinRows;
 }
 
 // Partial re-orthogonalisation (default), or a pass over the whole basis at every step
@@ -521,7 +445,7 @@ template <typename T>
 class ThickLanczos {
    public:
     ThickLanczos(sed_sector* sec, int m, Workspace& ws)
-        : sec_(sec), n_(sec->dim), m_(m), ws_(ws), st_(ws.stream), mfma_(reorth_mfma()), pro_on_(pro_enabled()) {}
+        : sec_(sec), n_(sec->dim), m_(m), ws_(ws), st_(ws.stream), mfma_(reorth_mfma(sec->dim)), pro_on_(pro_enabled()) {}
 
     int init() {
         SED_TRY(VecTraits<T>::reserve(ws_, n_, m_));

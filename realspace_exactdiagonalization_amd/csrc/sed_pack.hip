@@ -4,10 +4,12 @@
 // with KrylovKit (diagonalize_block_matrixfree, src/symmetry_resolved_ed.jl:998-1030)
 // inside the flux loop of ed_scan! (:1297-1356).  Here one workgroup of 1024 threads
 // owns one sector and runs whole thick-restart Lanczos steps inside the kernel:
-// H|psi> over all of the sector's rows, classical Gram-Schmidt against the stored basis
-// (a second pass only when the first one cancelled all but a tenth of the norm, the
-// Daniel-Gragg-Kaufman-Stewart test), normalisation and the next prescaled vector,
-// separated by workgroup barriers instead of kernel boundaries.  Every sector of a
+// H|psi> over all of the sector's rows, classical Gram-Schmidt against the stored basis,
+// normalisation and the next prescaled vector,
+// separated by workgroup barriers instead of kernel boundaries.  The Gram-Schmidt
+// passes follow the per-sector solver's partial re-orthogonalisation (a local pass
+// against v_{j-1}, v_j; the whole basis only when Simon's recurrence, a forced step or
+// cancellation asks for it; state per request in device memory).  Every sector of a
 // launch advances to its next check point; the host then reads all alpha/beta in one
 // copy, diagonalises the projected matrices in parallel, and queues the resulting
 // thick restarts and FP32 -> FP64 hand-overs as one batched kernel in front of the next
@@ -35,7 +37,7 @@ constexpr int kPackCheck = 8;             // steps between host checks
 constexpr int64_t kPackMaxDim = 1 << 15;   // one CU per sector: larger sectors go to the lanes
 constexpr int kPackDots = 4;              // basis vectors per sweep of the dot products
 constexpr int kMoveRows = 64;             // rows per workgroup of the restart kernel
-constexpr int kSlotDoubles = 2 * kMaxBasis + 2;   // alpha[m], beta[m], unused, (int step, int flag)
+constexpr int kSlotDoubles = 2 * kMaxBasis + 2;   // alpha[m], beta[m], full steps, (int step, int flag)
 
 enum PackStart { kStartNone = 0, kStartRandom = 1, kStartGiven = 2 };
 
@@ -54,6 +56,11 @@ struct PackItem {
     int target;     // run until the step index reaches target
     int start;      // PackStart: draw (kStartRandom) or take (kStartGiven) V_0, normalise, step 0
     int set_step;   // >= 0: continue at this step (after a thick restart)
+    ProState* pro;              // partial re-orthogonalisation state of the request
+    const double* pro_init;     // after a thick restart: kept Ritz values, then arrowhead
+    int pro_kk;                 // kept Ritz vectors of that restart (0: none)
+    int pro_enabled;
+    int pro_reset;              // first launch of the request: zero the full-step count
 };
 
 // One thick restart (kk > 0): V_i <- sum_{a<jm} V_a S[a*kk+i] for i < kk, then
@@ -166,7 +173,9 @@ __global__ void __launch_bounds__(kPackBlock) k_pack_lanczos(const PackItem* __r
     __shared__ double2 sh_red[kPackDots][NT / kWave];
     __shared__ double sh_redd[NT / kWave];
     __shared__ double sh_nrm;
+    __shared__ int sh_lo, sh_need;
     const PackItem& it = items[blockIdx.x];
+    ProState* pro = it.pro;
     const MvLds<R> L = stage_mv<R>(it.c, NT);
     T* V = reinterpret_cast<T*>(it.V);
     T* w = reinterpret_cast<T*>(it.w);
@@ -203,12 +212,22 @@ __global__ void __launch_bounds__(kPackBlock) k_pack_lanczos(const PackItem* __r
         }
         __syncthreads();
         j = 0;
-        if (threadIdx.x == 0) state[1] = 0;
+        if (threadIdx.x == 0) {
+            state[1] = 0;
+            pro_start(pro, sizeof(R) == 8, it.pro_enabled != 0);
+        }
     } else if (it.set_step >= 0) {
         j = it.set_step;
+        if (threadIdx.x == 0 && it.pro_kk > 0)
+            pro_restart(pro, it.pro_kk, it.pro_init, it.pro_init + it.pro_kk, alpha, beta);
     } else {
         j = state[0];
     }
+    if (threadIdx.x == 0) {
+        if (it.pro_reset) pro->nfull = 0;
+        sh_lo = pro->lo;
+    }
+    __syncthreads();
     const int target = it.target;
     while (j < target) {
         // w = H' v_j from xhat = sqrt(stab) v_j
@@ -224,13 +243,16 @@ __global__ void __launch_bounds__(kPackBlock) k_pack_lanczos(const PackItem* __r
             w[r] = out;
         }
         __syncthreads();
+        // first pass over v_lo..v_j, then the whole basis when pro_decide asks for it
+        const int lo = sh_lo;
         const int nv = j + 1;
-        pack_dots<T, NT>(V, n, nv, w, sh_h, &sh_nrm, sh_red);
+        pack_dots<T, NT>(V + (int64_t)lo * n, n, nv - lo, w, sh_h + lo, &sh_nrm, sh_red);
         const double nw = sh_nrm;
         double a = sh_h[j].x;
-        double nw2 = pack_update<T, NT>(V, n, nv, sh_h, w, sh_redd);
-        if (!(nw2 > kReorthEta2 * nw)) {
-            // the first pass cancelled most of w: orthogonalise once more
+        double nw2 = pack_update<T, NT>(V + (int64_t)lo * n, n, nv - lo, sh_h + lo, w, sh_redd);
+        if (threadIdx.x == 0) sh_need = pro_decide(pro, j, nw2, nw, a, alpha, beta) ? 1 : 0;
+        __syncthreads();
+        if (sh_need) {
             pack_dots<T, NT>(V, n, nv, w, sh_h, &sh_nrm, sh_red);
             a += sh_h[j].x;
             nw2 = pack_update<T, NT>(V, n, nv, sh_h, w, sh_redd);
@@ -242,6 +264,8 @@ __global__ void __launch_bounds__(kPackBlock) k_pack_lanczos(const PackItem* __r
             alpha[j] = a;
             beta[j] = b;
             if (brk && state[1] == 0) state[1] = j + 1;
+            pro_advance(pro, j, a, b, beta);
+            sh_lo = pro->lo;
         }
         T* vn = V + (int64_t)(j + 1) * n;
         for (int64_t r = threadIdx.x; r < n; r += NT) {
@@ -256,7 +280,10 @@ __global__ void __launch_bounds__(kPackBlock) k_pack_lanczos(const PackItem* __r
         ++j;
         if (brk) break;
     }
-    if (threadIdx.x == 0) state[0] = j;
+    if (threadIdx.x == 0) {
+        state[0] = j;
+        it.slot[2 * m] = (double)pro->nfull;
+    }
 }
 
 // Thick restarts and FP32 -> FP64 hand-overs of many sectors: grid (row tiles, moves).
@@ -391,6 +418,9 @@ struct PackReq {
     int j = 0;               // device step index at the last check
     int start = kStartNone;  // pending start of the next launch
     int set_step = -1;       // pending restart step of the next launch
+    int64_t pro_init_off = -1;   // pending restart data (Ritz values, arrowhead) in the coefficient buffer
+    int pro_kk = 0;
+    bool fresh = true;       // no launch yet
     int iters[2] = {0, 0};
     int cycles[2] = {0, 0};  // thick restarts per precision phase
     int restarts = 0;
@@ -412,7 +442,10 @@ struct Decision {
 class PackEngine {
    public:
     PackEngine(sed_context* ctx, int max_active, void (*cb)(void*, PackJob*), void* user)
-        : ctx_(ctx), max_active_(max_active), cb_(cb), user_(user) {}
+        : ctx_(ctx), max_active_(max_active), cb_(cb), user_(user) {
+        const char* e = std::getenv("SED_PRO");   // SED_PRO=0: the whole basis at every step
+        pro_on_ = !(e && e[0] == '0');
+    }
 
     int start(int64_t arena_bytes) {
         SED_HIP(hipSetDevice(ctx_->device));
@@ -420,6 +453,7 @@ class PackEngine {
         SED_TRY(arena_.init(arena_bytes));
         const size_t coef_doubles = (size_t)max_active_ * kMaxBasis * kMaxBasis;
         SED_TRY(slots_.alloc((size_t)max_active_ * kSlotDoubles));
+        SED_TRY(pros_.alloc((size_t)max_active_ * sizeof(ProState)));
         SED_TRY(items_.alloc((size_t)max_active_ * sizeof(PackItem)));
         SED_TRY(moves_.alloc((size_t)max_active_ * sizeof(PackMove)));
         SED_TRY(coef_.alloc(coef_doubles));
@@ -454,6 +488,7 @@ class PackEngine {
         if (h_moves_) (void)hipHostFree(h_moves_);
         if (h_coef_) (void)hipHostFree(h_coef_);
         slots_.release();
+        pros_.release();
         items_.release();
         moves_.release();
         coef_.release();
@@ -462,6 +497,7 @@ class PackEngine {
 
    private:
     double* slot_ptr(int s) { return slots_.p + (size_t)s * kSlotDoubles; }
+    ProState* pro_ptr(int s) { return reinterpret_cast<ProState*>(pros_.p) + s; }
 
     void finish(PackReq& q, int rc, const std::string& err) {
         PackJob* job = q.job;
@@ -470,7 +506,7 @@ class PackEngine {
         job->stats.iters_fp32 = q.iters[0];
         job->stats.iters_fp64 = q.iters[1];
         job->stats.matvecs = q.iters[0] + q.iters[1];
-        job->stats.full_orth_steps = q.iters[0] + q.iters[1];   // every packed step spans the basis
+        job->stats.full_orth_steps = q.slot >= 0 ? (int)h_slots_[(size_t)q.slot * kSlotDoubles + 2 * q.m] : 0;
         job->stats.restarts = q.restarts;
         job->stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - q.t0).count();
         if (q.off >= 0) arena_.release(q.off, q.bytes);
@@ -510,6 +546,8 @@ class PackEngine {
         q.phase = fp32 ? 0 : 1;
         q.start = kStartRandom;
         q.set_step = -1;
+        q.pro_kk = 0;
+        q.fresh = true;
         q.T.assign((size_t)q.m * q.m, 0.0);
         q.k = 0;
         q.j = 0;
@@ -645,6 +683,16 @@ class PackEngine {
                     q.j = d.kk;
                     ++q.restarts;
                     ++q.cycles[q.phase];
+                    // kept Ritz values and their couplings to v_kk for the device's
+                    // re-orthogonalisation state
+                    double* h = h_coef_ + coef_used;
+                    for (int i = 0; i < d.kk; ++i) {
+                        h[i] = d.theta[i];
+                        h[d.kk + i] = q.T[(size_t)d.kk * q.m + i];
+                    }
+                    q.pro_init_off = (int64_t)coef_used;
+                    q.pro_kk = d.kk;
+                    coef_used += (2 * (size_t)d.kk + 31) & ~(size_t)31;
                 }
                 moves.push_back({mv, f32 ? 0 : 1});
                 return 0;
@@ -680,6 +728,13 @@ class PackEngine {
             it.m = q.m;
             it.start = q.start;
             it.set_step = q.set_step;
+            it.pro = pro_ptr(q.slot);
+            it.pro_kk = q.set_step >= 0 ? q.pro_kk : 0;
+            it.pro_init = it.pro_kk > 0 ? coef_.p + q.pro_init_off : nullptr;
+            it.pro_enabled = pro_on_ ? 1 : 0;
+            it.pro_reset = q.fresh ? 1 : 0;
+            q.fresh = false;
+            q.pro_kk = 0;
             const int j0 = q.start != kStartNone ? 0 : (q.set_step >= 0 ? q.set_step : q.j);
             const int first = q.job->nev + 2;
             it.target = std::min(q.m, j0 < first ? first : j0 + kPackCheck);
@@ -847,6 +902,7 @@ class PackEngine {
     bool stop_ = false;
     Arena arena_;
     DevBuf<double> slots_;
+    DevBuf<uint8_t> pros_;      // ProState per slot
     DevBuf<uint8_t> items_, moves_;
     DevBuf<double> coef_;
     double* h_slots_ = nullptr;
@@ -854,6 +910,7 @@ class PackEngine {
     PackMove* h_moves_ = nullptr;
     double* h_coef_ = nullptr;
     std::vector<int> free_slots_;
+    bool pro_on_ = true;
 };
 
 PackEngine* pack_create(sed_context* ctx, int64_t arena_bytes, int max_active, void (*on_done)(void*, PackJob*),

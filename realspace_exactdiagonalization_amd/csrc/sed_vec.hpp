@@ -34,6 +34,128 @@ __device__ __forceinline__ bool pass_range(const int* step, const int* lo, int d
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Partial re-orthogonalisation (Simon): state in device memory, one per Krylov process.
+// omega[r % 3][k] estimates |v_r^H v_k| for the rows r = j-1, j, j+1 around step j; T is
+// the projected matrix: alpha[k] on the diagonal (the kept Ritz values after a thick
+// restart), beta[k] = T[k+1][k] for k >= kk, and the restart's arrowhead
+// T[kk][k] = arrow[k] for k < kk.
+// ---------------------------------------------------------------------------
+struct ProState {
+    int lo;        // first basis vector of the current step's first pass
+    int force;     // steps, from the current one, that run the second pass in any case
+    int kk;        // kept Ritz vectors of this cycle (the arrowhead row)
+    int nfull;     // steps with a pass over the whole basis
+    int enabled;   // 0: every pass spans the whole basis
+    int pad[3];
+    double eps;     // unit roundoff of the stored vectors
+    double delta;   // semi-orthogonality threshold, sqrt(eps)
+    double anorm;   // running estimate of |T|
+    double pad2;
+    double arrow[kMaxBasis];
+    double omega[3][kMaxBasis + 2];
+};
+
+// Start of a Krylov process (step 0): first passes span the basis until step 2.
+__device__ __forceinline__ void pro_start(ProState* pro, bool fp64, bool enabled) {
+    pro->lo = 0;
+    pro->force = 0;
+    pro->kk = 0;
+    pro->enabled = enabled ? 1 : 0;
+    pro->eps = fp64 ? 1.1102230246251565e-16 : 5.9604644775390625e-08;   // 2^-53, 2^-24
+    pro->delta = sqrt(pro->eps);
+    pro->anorm = 0.0;
+}
+
+// After the first Gram-Schmidt pass of step j (over v_lo..v_j, lo = pro->lo): whether the
+// second pass, over the whole basis, must run.  It runs when the first one cancelled most
+// of w (|w'|^2 <= eta^2 |w|^2, Daniel-Gragg-Kaufman-Stewart), when the step is forced, or
+// when the first pass was local (lo > 0) and the recurrence
+//   beta_j omega_{j+1,k} = (T omega_j)_k - beta_{j-1} omega_{j-1,k} - alpha_j omega_{j,k}
+//                          +- eps |T|   (rounding of the step)
+// puts some |omega_{j+1,k}|, k <= j-2, above sqrt(eps).  The step after such a step is
+// forced too (v_j itself is only semi-orthogonal, and its large coefficient in the next
+// vector carries that into it), as are the two steps after a thick restart.  A forced
+// step keeps its local first pass, which removes the large components, so that the
+// second pass over the semi-orthogonal basis leaves eps-level overlaps.  After a second
+// pass (or a first pass over an orthonormal basis, steps 0 and 1) omega_{j+1,.} is reset
+// to eps; the two vectors a local pass orthogonalises against get eps.
+// nrm2 = |w'|^2 after the first pass, nw = |w|^2 before it, aj = Re h_j of the first pass.
+// One thread.
+__device__ inline bool pro_decide(ProState* pro, int j, double nrm2, double nw, double aj,
+                                  const double* alpha, const double* beta) {
+    bool need = !(nrm2 > kReorthEta2 * nw);
+    if (pro->force > 0) {
+        need = true;
+        --pro->force;
+    }
+    const int lo = pro->lo;
+    double* wn = pro->omega[(j + 1) % 3];
+    const double eps = pro->eps;
+    if (pro->enabled && lo > 0 && !need) {
+        const double* wj = pro->omega[j % 3];
+        const double* wm = pro->omega[(j + 2) % 3];   // row j - 1
+        const int kk = pro->kk;
+        const double bj = sqrt(nrm2);
+        const double bjm = beta[j - 1];
+        const double noise = eps * fmax(pro->anorm, fabs(aj) + bj + bjm);
+        double mx = 0.0;
+        for (int k = 0; k <= j - 2; ++k) {
+            double tw = alpha[k] * wj[k];
+            if (k < kk) {
+                tw += pro->arrow[k] * wj[kk];
+            } else {
+                tw += beta[k] * wj[k + 1];
+                if (k > kk) {
+                    tw += beta[k - 1] * wj[k - 1];
+                } else {
+                    for (int b = 0; b < kk; ++b) tw += pro->arrow[b] * wj[b];
+                }
+            }
+            double x = tw - bjm * wm[k] - aj * wj[k];
+            x += x < 0.0 ? -noise : noise;
+            x /= bj;
+            wn[k] = x;
+            mx = fmax(mx, fabs(x));
+        }
+        if (!(mx <= pro->delta)) {   // NaN (bj = 0) included
+            need = true;
+            pro->force = pro->force > 1 ? pro->force : 1;
+        } else {
+            wn[j - 1] = eps;
+            wn[j] = eps;
+        }
+    }
+    if (need || lo == 0) {
+        for (int k = 0; k <= j; ++k) wn[k] = eps;
+        ++pro->nfull;
+    }
+    wn[j + 1] = 1.0;
+    return need;
+}
+
+// End of step j (alpha_j = a, beta_j = b final): the next step's first pass covers
+// v_j, v_{j+1} (the whole basis when partial re-orthogonalisation is off).  One thread.
+__device__ __forceinline__ void pro_advance(ProState* pro, int j, double a, double b, const double* beta) {
+    pro->anorm = fmax(pro->anorm, fabs(a) + b + (j > 0 ? beta[j - 1] : 0.0));
+    pro->lo = pro->enabled ? j : 0;
+}
+
+// After a thick restart to kk kept Ritz vectors (values theta, arrowhead couplings arrow):
+// the diagonal of T, no coupling among them, step kk's first pass over the whole basis
+// (the arrowhead), and the second pass at steps kk and kk + 1.  One thread.
+__device__ __forceinline__ void pro_restart(ProState* pro, int kk, const double* theta, const double* arrow,
+                                            double* alpha, double* beta) {
+    for (int i = 0; i < kk; ++i) {
+        alpha[i] = theta[i];
+        beta[i] = 0.0;
+        pro->arrow[i] = arrow[i];
+    }
+    pro->lo = 0;
+    pro->force = pro->enabled ? 2 : 0;
+    pro->kk = kk;
+}
+
 // Krylov basis size for a sector of dimension n and nev wanted eigenpairs.
 inline int krylov_dim(int64_t n, int nev) {
     int m = std::max(2 * nev + 24, 40);

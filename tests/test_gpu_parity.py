@@ -346,11 +346,12 @@ def test_packed_engine_matches_lane_path(ed, ctx, monkeypatch):
 
 
 def test_mfma_reorthogonalisation_matches_vector_loops(ed, ctx, oracle_mod, monkeypatch):
-    """The Lanczos re-orthogonalisation and restart rotation on the FP64 matrix cores give
-    the eigenvalues of the row-wise VALU loops (SED_REORTH=valu) to 1e-12 relative, both
-    match the oracle to 1e-10, and the MFMA path is bitwise reproducible.  The sector
-    (checkerboard 3x4, 8 fermions, dim 61,333) needs several thick restarts and basis
-    sizes on both sides of every 16-vector tile boundary."""
+    """The Lanczos re-orthogonalisation and restart rotation on the FP64 matrix cores
+    (forced with SED_REORTH=mfma: this sector is below the size from which they are the
+    default) give the eigenvalues of the row-wise VALU loops (SED_REORTH=valu) to 1e-12
+    relative, both match the oracle to 1e-10, and the MFMA path is bitwise reproducible.
+    The sector (checkerboard 3x4, 8 fermions, dim 61,333) needs several thick restarts and
+    basis sizes on both sides of every 16-vector tile boundary."""
     m, G = make("checker3x4_f")
     ir = L.translation_irreps(G, m.lattice)[0]
     oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, True)
@@ -364,10 +365,7 @@ def test_mfma_reorthogonalisation_matches_vector_loops(ed, ctx, oracle_mod, monk
     print(f"[reorth] checker3x4 dim={n}", flush=True)
     res = {}
     for mode in ("mfma", "valu", "mfma"):
-        if mode == "valu":
-            monkeypatch.setenv("SED_REORTH", "valu")
-        else:
-            monkeypatch.delenv("SED_REORTH", raising=False)
+        monkeypatch.setenv("SED_REORTH", mode)
         for prec in (ed.PREC_FP64, ed.PREC_FP32_REFINE):
             st = {}
             vals = ed.diagonalize_block_matrixfree(H, n, nev=4, precision=prec, return_vectors=False, stats=st)[0]
@@ -392,15 +390,19 @@ PRO_CASES = {
 }
 
 
-@pytest.mark.parametrize("name", list(PRO_CASES))
-def test_partial_reorthogonalisation_matches_full(ed, ctx, oracle_mod, monkeypatch, name):
+@pytest.mark.parametrize("name,reorth", [("checker3x4_f", "mfma"), ("checker3x4_f", None),
+                                         ("hald3x4_flux_nev8", None), ("xxz20", None)])
+def test_partial_reorthogonalisation_matches_full(ed, ctx, oracle_mod, monkeypatch, name, reorth):
     """Partial re-orthogonalisation (the default: Simon's recurrence decides when a step is
     orthogonalised against the whole basis) against a pass over the whole basis at every
     step (SED_PRO=0): the same eigenvalues to 1e-11 relative in FP64 and FP32+refine, both
     within 1e-10 of the oracle, the same number of Lanczos steps within 10 %, at most half
     as many steps orthogonalised against the whole basis, and bitwise reproducible runs.
-    Sectors with several thick restarts: checkerboard 3x4 fermions (dim 61,333), the
-    twisted Haldane 3x4 bosons with nev = 8 (dim ~11,200), XXZ N=20 (dim 9,252)."""
+    Sectors with several thick restarts: checkerboard 3x4 fermions (dim 61,333; also with
+    the matrix-core kernels forced, SED_REORTH=mfma, which large sectors use by default),
+    the twisted Haldane 3x4 bosons with nev = 8 (dim ~11,200), XXZ N=20 (dim 9,252)."""
+    if reorth:
+        monkeypatch.setenv("SED_REORTH", reorth)
     f, th, k_idx, nev = PRO_CASES[name]
     m = L.twisted(f(), th)
     G = L.model_group(m, th)
