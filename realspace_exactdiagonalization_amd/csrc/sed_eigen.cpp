@@ -12,10 +12,20 @@
 
 namespace sed {
 
+// sqrt(x^2 + y^2): the plain form where it can neither overflow nor underflow, hypot
+// elsewhere (hypot costs several times more, and QL calls this once per rotation).
+static inline double pyth(double x, double y) {
+    const double r = std::sqrt(x * x + y * y);
+    return (r > 1e-150 && r < 1e150) ? r : std::hypot(x, y);
+}
+
 // Implicit QL on a real symmetric tridiagonal matrix (diagonal d, sub-diagonal e with
-// e[i] coupling i and i+1).  Eigenvectors are accumulated into z (n x n, column j is
-// the j-th vector, row-major storage z[i*n + j]) when z is non-null.
-static void tridiag_ql(int n, std::vector<double>& d, std::vector<double>& e, std::vector<double>* z) {
+// e[i] coupling i and i+1).  The rotations are applied from the right to z (zrows x n,
+// row-major) when z is non-null: with z = Q (n x n) it ends as the eigenvector matrix
+// (column j = eigenvector j); with z = a few rows of Q it ends as those rows of it.
+static void tridiag_ql(int n, std::vector<double>& d, std::vector<double>& e, std::vector<double>* z,
+                       int zrows = -1) {
+    if (zrows < 0) zrows = n;
     if (n <= 1) return;
     e.resize(n);
     e[n - 1] = 0.0;
@@ -31,14 +41,14 @@ static void tridiag_ql(int n, std::vector<double>& d, std::vector<double>& e, st
             if (++iter > 200) break;
             // Wilkinson-type shift from the leading 2x2 block
             double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = std::hypot(g, 1.0);
+            double r = pyth(g, 1.0);
             g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
             double s = 1.0, c = 1.0, p = 0.0;
             int i = m - 1;
             for (; i >= l; --i) {
                 double f = s * e[i];
                 const double b = c * e[i];
-                r = std::hypot(f, g);
+                r = pyth(f, g);
                 e[i + 1] = r;
                 if (r == 0.0) {
                     d[i + 1] -= p;
@@ -53,7 +63,7 @@ static void tridiag_ql(int n, std::vector<double>& d, std::vector<double>& e, st
                 d[i + 1] = g + p;
                 g = c * r - b;
                 if (z) {
-                    for (int k = 0; k < n; ++k) {
+                    for (int k = 0; k < zrows; ++k) {
                         f = (*z)[(size_t)k * n + i + 1];
                         (*z)[(size_t)k * n + i + 1] = s * (*z)[(size_t)k * n + i] + c * f;
                         (*z)[(size_t)k * n + i] = c * (*z)[(size_t)k * n + i] - s * f;
@@ -152,13 +162,13 @@ void hermitian_eigh(int n, std::vector<cd>& A, std::vector<double>& evals, std::
     }
 }
 
-// Real symmetric n x n (row-major a, destroyed): Householder reduction to tridiagonal
-// form with the orthogonal factor accumulated, then implicit QL on it.  Eigenvalues
-// ascending; evecs row-major, column j = eigenvector j.
-void jacobi_eigh(int n, std::vector<double>& a, std::vector<double>& evals, std::vector<double>& evecs) {
+// Householder reduction of a real symmetric n x n matrix (row-major a, destroyed) to
+// tridiagonal form d, e: A = Q T Q^T with Q = H_0 H_1 ..., H_k = I - tau_k v_k v_k^T
+// (tau_k = 0 where column k was already reduced).
+static void householder_tridiag(int n, std::vector<double>& a, std::vector<std::vector<double>>& vs,
+                                std::vector<double>& taus, std::vector<double>& d, std::vector<double>& e) {
     auto A = [&](int i, int j) -> double& { return a[(size_t)i * n + j]; };
-    std::vector<std::vector<double>> vs;
-    std::vector<double> taus, v(n), p(n), q(n);
+    std::vector<double> v(n), p(n), q(n);
     for (int k = 0; k + 2 < n; ++k) {
         double sigma2 = 0.0;
         for (int i = k + 1; i < n; ++i) sigma2 += A(i, k) * A(i, k);
@@ -191,9 +201,26 @@ void jacobi_eigh(int n, std::vector<double>& a, std::vector<double>& evals, std:
         vs.push_back(v);
         taus.push_back(tau);
     }
-    std::vector<double> d(n), e(std::max(n, 1), 0.0);
+    d.assign(n, 0.0);
+    e.assign(std::max(n, 1), 0.0);
     for (int i = 0; i < n; ++i) d[i] = A(i, i);
     for (int i = 0; i + 1 < n; ++i) e[i] = A(i + 1, i);
+}
+
+static std::vector<int> ascending(const std::vector<double>& d) {
+    std::vector<int> order(d.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return d[x] < d[y]; });
+    return order;
+}
+
+// Real symmetric n x n (row-major a, destroyed): Householder reduction to tridiagonal
+// form with the orthogonal factor accumulated, then implicit QL on it.  Eigenvalues
+// ascending; evecs row-major, column j = eigenvector j.
+void jacobi_eigh(int n, std::vector<double>& a, std::vector<double>& evals, std::vector<double>& evecs) {
+    std::vector<std::vector<double>> vs;
+    std::vector<double> taus, d, e;
+    householder_tridiag(n, a, vs, taus, d, e);
     // Q = H_0 H_1 ... applied to the identity (row-major)
     std::vector<double> z((size_t)n * n, 0.0);
     for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;
@@ -208,14 +235,41 @@ void jacobi_eigh(int n, std::vector<double>& a, std::vector<double>& evals, std:
         }
     }
     tridiag_ql(n, d, e, &z);
-    std::vector<int> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int x, int y) { return d[x] < d[y]; });
+    const std::vector<int> order = ascending(d);
     evals.resize(n);
     evecs.assign((size_t)n * n, 0.0);
     for (int j = 0; j < n; ++j) {
         evals[j] = d[order[j]];
         for (int i = 0; i < n; ++i) evecs[(size_t)i * n + j] = z[(size_t)i * n + order[j]];
+    }
+}
+
+// Eigenvalues (ascending) of a real symmetric n x n matrix (row-major a, destroyed) and
+// the last row of its eigenvector matrix, last[j] = component n-1 of eigenvector j (the
+// Lanczos residual estimates): the same reduction and QL iteration as jacobi_eigh, with
+// only the row e_{n-1}^T Q carried through the reflectors and rotations, O(n^2) per QL
+// sweep instead of O(n^3).
+void eigh_last_row(int n, std::vector<double>& a, std::vector<double>& evals, std::vector<double>& last) {
+    std::vector<std::vector<double>> vs;
+    std::vector<double> taus, d, e;
+    householder_tridiag(n, a, vs, taus, d, e);
+    std::vector<double> z((size_t)n, 0.0);   // e_{n-1}^T H_0 H_1 ...
+    if (n > 0) z[(size_t)n - 1] = 1.0;
+    for (size_t k = 0; k < vs.size(); ++k) {
+        if (taus[k] == 0.0) continue;
+        const std::vector<double>& vk = vs[k];
+        double sm = 0.0;
+        for (int i = (int)k + 1; i < n; ++i) sm += z[i] * vk[i];
+        sm *= taus[k];
+        for (int i = (int)k + 1; i < n; ++i) z[i] -= sm * vk[i];
+    }
+    tridiag_ql(n, d, e, &z, 1);
+    const std::vector<int> order = ascending(d);
+    evals.resize(n);
+    last.resize(n);
+    for (int j = 0; j < n; ++j) {
+        evals[j] = d[order[j]];
+        last[j] = z[order[j]];
     }
 }
 

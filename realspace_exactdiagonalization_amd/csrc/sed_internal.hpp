@@ -237,6 +237,7 @@ inline bool fp32_cluster(const std::vector<double>& theta, int nev) {
 // ---- host eigensolvers (sed_eigen.cpp) ----
 // Real symmetric n x n (row-major, overwritten) -> ascending eigenvalues, eigenvectors in columns.
 void jacobi_eigh(int n, std::vector<double>& a, std::vector<double>& evals, std::vector<double>& evecs);
+void eigh_last_row(int n, std::vector<double>& a, std::vector<double>& evals, std::vector<double>& last);
 // Complex Hermitian n x n (column-major) -> ascending eigenvalues and eigenvectors (columns).
 void hermitian_eigh(int n, std::vector<cd>& a, std::vector<double>& evals, std::vector<cd>* evecs);
 

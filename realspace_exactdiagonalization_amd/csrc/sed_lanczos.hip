@@ -506,7 +506,7 @@ class ThickLanczos {
         int k = 0;
         double beta = 0.0;
         out = SolveOut();
-        std::vector<double> theta, Sm;
+        std::vector<double> theta, Sm, last;
         int jm = 0;
         bool done = false, breakdown = false;
         // first matvec eagerly: it fixes the launch shape (autotuner) before any capture
@@ -541,10 +541,12 @@ class ThickLanczos {
                     }
                 }
                 beta = breakdown ? 0.0 : hb[jm - 1];
-                eig(jm, T_, theta, Sm);
+                // eigenvalues and the last row of the eigenvectors decide convergence; the
+                // whole eigenvector matrix only where Ritz vectors are formed
+                eig_last(jm, T_, theta, last);
                 int conv = 0;
                 for (int i = 0; i < std::min(nev, jm); ++i) {
-                    const double res = beta * std::fabs(Sm[(size_t)(jm - 1) * jm + i]);
+                    const double res = beta * std::fabs(last[i]);
                     if (res <= tol) ++conv; else break;
                 }
                 if (breakdown || conv >= nev || (j == m_ && out.restarts >= maxiter)) {
@@ -552,6 +554,7 @@ class ThickLanczos {
                     break;
                 }
             }
+            eig(jm, T_, theta, Sm);
             if (done) break;
             // thick restart: kk Ritz vectors + the residual direction v_m
             const int kk = std::min(m_ - 2, nev + std::max(4, (m_ - nev) / 3));
@@ -578,8 +581,8 @@ class ThickLanczos {
         out.evals.assign(theta.begin(), theta.begin() + nk);
         out.resid.resize(nk);
         out.converged = 0;
-        for (int i = 0; i < nk; ++i) {
-            out.resid[i] = beta * std::fabs(Sm[(size_t)(jm - 1) * jm + i]);
+        for (int i = 0; i < nk; ++i) {   // the estimates the last check decided on
+            out.resid[i] = beta * std::fabs(last[i]);
             if (out.resid[i] <= tol && out.converged == i) ++out.converged;
         }
         std::vector<double> Sk((size_t)jm * nk);
@@ -731,6 +734,13 @@ class ThickLanczos {
         for (int i = 0; i < jm; ++i)
             for (int j = 0; j < jm; ++j) a[(size_t)i * jm + j] = T_[(size_t)i * m + j];
         jacobi_eigh(jm, a, theta, S);
+    }
+    static void eig_last(int jm, const std::vector<double>& T_, std::vector<double>& theta, std::vector<double>& last) {
+        const int m = (int)std::lround(std::sqrt((double)T_.size()));
+        std::vector<double> a((size_t)jm * jm);
+        for (int i = 0; i < jm; ++i)
+            for (int j = 0; j < jm; ++j) a[(size_t)i * jm + j] = T_[(size_t)i * m + j];
+        eigh_last_row(jm, a, theta, last);
     }
 
     sed_sector* sec_;

@@ -579,15 +579,25 @@ class PackEngine {
             }
         }
         const double beta = breakdown ? 0.0 : hb[jm - 1];
-        std::vector<double> a((size_t)jm * jm), Sm;
-        for (int r = 0; r < jm; ++r)
-            for (int c = 0; c < jm; ++c) a[(size_t)r * jm + c] = q.T[(size_t)r * m + c];
-        jacobi_eigh(jm, a, d.theta, Sm);
+        std::vector<double> a((size_t)jm * jm), Sm, last;
+        auto load = [&] {
+            for (int r = 0; r < jm; ++r)
+                for (int c = 0; c < jm; ++c) a[(size_t)r * jm + c] = q.T[(size_t)r * m + c];
+        };
+        // eigenvalues and the last eigenvector row decide; the whole eigenvector matrix
+        // only for a hand-over or a thick restart
+        load();
+        eigh_last_row(jm, a, d.theta, last);
+        auto full = [&] {
+            load();
+            std::vector<double> th;
+            jacobi_eigh(jm, a, th, Sm);
+        };
         const int nev = std::min(job->nev, jm);
         const double tol = q.phase == 0 ? (job->precision == SED_PREC_FP32 ? std::max(job->tol, 1e-5) : 1e-4)
                                         : job->tol;
         for (int i = 0; i < nev; ++i) {
-            const double res = beta * std::fabs(Sm[(size_t)(jm - 1) * jm + i]);
+            const double res = beta * std::fabs(last[i]);
             if (res <= tol && d.conv == i) ++d.conv;
             d.resmax = std::max(d.resmax, res);
         }
@@ -601,6 +611,7 @@ class PackEngine {
                 d.action = 4;   // degenerate FP32 Ritz values: FP64 phase from the seeded random vector
             } else if (q.phase == 0 && job->precision == SED_PREC_FP32_REFINE) {
                 // FP64 start vector = sum of the wanted FP32 Ritz vectors
+                full();
                 d.action = 2;
                 d.kk = 0;
                 d.coef.assign(jm, 0.0);
@@ -618,6 +629,7 @@ class PackEngine {
             return;
         }
         if (jm == m) {
+            full();
             d.action = 3;
             const int kk = std::min(m - 2, job->nev + std::max(4, (m - job->nev) / 3));
             d.kk = kk;
