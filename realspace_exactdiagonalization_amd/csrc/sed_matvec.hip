@@ -369,21 +369,25 @@ static int run_any_shape(sed_sector* sec, int shape, int chunk, const typename C
 
 // Autotuner: on the first matvec of a sector large enough for the launch shape to
 // matter, time every shape (XCD run length 1) on the call's own buffers, then every XCD
-// run length for the fastest shape, then every shape at the fastest run length
-// (deterministic kernels, so the repeated launches leave y as one launch would), and
-// keep the fastest pair.  Each candidate's time is the minimum over three separate
-// launches, so work of other streams running meanwhile inflates it less.  A
-// process-wide cache keyed by the model's shape parameters and the sector dimension
-// rounded to a quarter octave skips re-tuning (all momentum sectors of a model share it).
+// run length for the fastest shape, then the shapes that came within 20 % of the fastest
+// at the fastest run length (deterministic kernels, so the repeated launches leave y as
+// one launch would), and keep the fastest pair.  A candidate whose first timed launch is
+// within 10 % of the best so far is timed twice more and keeps the minimum, so work of
+// other streams running meanwhile inflates it less; a slower one keeps its single time.
+// Each kernel instantiation gets one untimed warm-up launch.  A process-wide cache keyed
+// by the model's shape parameters and the sector dimension rounded to a quarter octave
+// skips re-tuning (all momentum sectors of a model share it).
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, int, int, int, int, int>, std::pair<int, int>> g_tune_cache;
 constexpr int64_t kTuneMinDim = 1 << 17;
 
+// Time one (shape, run length): an optional warm-up launch, one timed launch, and two more
+// when the first is within 10 % of `best` (the best time so far); the minimum.
 template <typename R>
 static int time_shape(sed_sector* sec, int shape, int chunk, const typename C2<R>::T* xh, typename C2<R>::T* y,
-                      hipStream_t s, hipEvent_t e0, hipEvent_t e1, float* ms_out) {
-    SED_TRY(run_any_shape<R>(sec, shape, chunk, xh, y, s));  // warm-up
-    float best = 1e30f;
+                      hipStream_t s, hipEvent_t e0, hipEvent_t e1, bool warm, float best, float* ms_out) {
+    if (warm) SED_TRY(run_any_shape<R>(sec, shape, chunk, xh, y, s));
+    float mine = 1e30f;
     for (int k = 0; k < 3; ++k) {
         SED_HIP(hipEventRecord(e0, s));
         SED_TRY(run_any_shape<R>(sec, shape, chunk, xh, y, s));
@@ -391,9 +395,10 @@ static int time_shape(sed_sector* sec, int shape, int chunk, const typename C2<R
         SED_HIP(hipEventSynchronize(e1));
         float ms = 0;
         SED_HIP(hipEventElapsedTime(&ms, e0, e1));
-        best = std::min(best, ms);
+        mine = std::min(mine, ms);
+        if (k == 0 && mine > 1.1f * best) break;
     }
-    *ms_out = best;
+    *ms_out = mine;
     return SED_OK;
 }
 
@@ -435,7 +440,7 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<
             sec->tune_ms[pi][i] = -1.0f;
             continue;
         }
-        SED_TRY(time_shape<R>(sec, i, 1, xh, y, s, e0, e1, &ms));
+        SED_TRY(time_shape<R>(sec, i, 1, xh, y, s, e0, e1, true, best, &ms));
         sec->tune_ms[pi][i] = ms;
         if (ms < best) {
             best = ms;
@@ -445,23 +450,24 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<
     for (int c = 0; c < kNumChunks; ++c) {
         float ms = 0;
         if (kChunks[c] == 1) ms = sec->tune_ms[pi][best_i];
-        else SED_TRY(time_shape<R>(sec, best_i, kChunks[c], xh, y, s, e0, e1, &ms));
+        else SED_TRY(time_shape<R>(sec, best_i, kChunks[c], xh, y, s, e0, e1, false, best, &ms));
         sec->tune_ms[pi][kNumShapes + c] = ms;
         if (ms < best) {
             best = ms;
             best_c = kChunks[c];
         }
     }
-    // the shapes once more at the chosen run length
+    // the close shapes once more at the chosen run length (-1: not fitting or not close)
     const int shape_c = best_i;
+    const float best1 = sec->tune_ms[pi][shape_c];
     for (int i = 0; i < kNumShapes; ++i) {
         float ms = 0;
-        if (!shape_fits<R>(sec, i)) {
+        if (!shape_fits<R>(sec, i) || !(sec->tune_ms[pi][i] <= 1.2f * best1)) {
             sec->tune_ms[pi][kNumShapes + kNumChunks + i] = -1.0f;
             continue;
         }
         if (i == shape_c) ms = sec->tune_ms[pi][kNumShapes + (int)(std::find(kChunks, kChunks + kNumChunks, best_c) - kChunks)];
-        else SED_TRY(time_shape<R>(sec, i, best_c, xh, y, s, e0, e1, &ms));
+        else SED_TRY(time_shape<R>(sec, i, best_c, xh, y, s, e0, e1, false, best, &ms));
         sec->tune_ms[pi][kNumShapes + kNumChunks + i] = ms;
         if (ms < best) {
             best = ms;
