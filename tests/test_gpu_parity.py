@@ -345,6 +345,59 @@ def test_packed_engine_matches_lane_path(ed, ctx, monkeypatch):
         assert np.abs(a - b).max() <= 1e-10 * np.abs(a).max()
 
 
+def test_packed_engine_partial_reorthogonalisation(ed, ctx, oracle_mod, monkeypatch):
+    """The packed persistent launches (SED_PACKED=1) with partial re-orthogonalisation give
+    the eigenvalues of a whole-basis pass at every step (SED_PRO=0) to 1e-11 relative and
+    the oracle's to 1e-10, FP64 and FP32+refine, with at most half of the steps
+    orthogonalised against the whole basis (every step without it).  Twisted Haldane 3x4
+    bosons and checkerboard 3x4 fermions, sectors of dimension 1,000-12,000."""
+    cases = [(L.haldane_honeycomb(3, 4), (0.3, 0.0)), (L.checkerboard(3, 4, n_filled=6), (0.0, 0.21))]
+    bases, refs, keep = [], [], []
+    for base, th in cases:
+        m = L.twisted(base, th)
+        G = L.model_group(m, th)
+        cat = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                              particle_statistics=m.statistics, ctx=ctx)
+        oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, m.statistics == L.FERMIONIC)
+        keep.append(cat)
+        for ir in L.translation_irreps(G, m.lattice)[:3]:
+            b = ed.build_symmetry_sector_basis(cat, ir)
+            if b.dim <= 500:
+                continue
+            b.set_terms(m.bilinear_terms, m.density_terms)
+            os_ = oracle_mod.Sector(oc, ir.values)
+            os_.set_terms(*L.term_arrays(m))
+            bases.append(b)
+            refs.append(oracle_mod.lowest_eigenvalues(os_, 3))
+    monkeypatch.setenv("SED_PACKED", "1")
+    out = {}
+    for mode in ("pro", "full"):
+        if mode == "full":
+            monkeypatch.setenv("SED_PRO", "0")
+        else:
+            monkeypatch.delenv("SED_PRO", raising=False)
+        srv = ed.SectorServer(ctx, max_slots=4)
+        ids = [(srv.submit(b, 3, precision=p), p) for b in bases for p in (ed.PREC_FP64, ed.PREC_FP32_REFINE)]
+        srv.run(0)
+        res = []
+        for rid, p in ids:
+            st, ev, stats, lat = srv.result(rid)
+            assert st == 0, (mode, rid)
+            steps = stats["iters_fp32"] + stats["iters_fp64"]
+            print(f"[packed-pro] {mode} prec={p} steps={steps} full={stats['full_orth_steps']}", flush=True)
+            if mode == "full":
+                assert stats["full_orth_steps"] == steps
+            else:
+                assert 0 < stats["full_orth_steps"] <= 0.5 * steps
+            res.append(ev)
+        srv.close()
+        out[mode] = res
+    for i, (a, b) in enumerate(zip(out["pro"], out["full"])):
+        ref = refs[i // 2]
+        assert np.abs(a - b).max() <= 1e-11 * np.abs(a).max()
+        assert np.abs(a - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
 def test_mfma_reorthogonalisation_matches_vector_loops(ed, ctx, oracle_mod, monkeypatch):
     """The Lanczos re-orthogonalisation and restart rotation on the FP64 matrix cores
     (forced with SED_REORTH=mfma: this sector is below the size from which they are the
