@@ -348,6 +348,44 @@ def solve_model(L, name):
     return L.workload(name)
 
 
+def reference_cases(args):
+    """The reference's own single-sector benchmark on its own models
+    (benchmark/benchmark.jl:226-320): per model, three repetitions of [build the orbit
+    catalog (untimed), then time ed_scan_at_irrep_matrixfree on the first momentum sector
+    with nev = 1], mean of repetitions 2-3 (the first warms caches; here it also autotunes).
+    One JSON line per model, with the reference's stored matrix-free and matrix-mode times
+    (benchmark/benchmark_data/benchmark_raw.csv, kept in tests/golden: CPU hardware and
+    thread count unrecorded) and the energy against its stored value."""
+    from realspace_exactdiagonalization_amd import ed as E
+    from realspace_exactdiagonalization_amd import lattice as L
+    with open(os.path.join(ROOT, "tests", "golden", "reference_values.json")) as f:
+        rows = json.load(f)["benchmark_raw"]
+    cases = []
+    for r in rows:
+        if (r["model"], r["label"]) not in cases:
+            cases.append((r["model"], r["label"]))
+    ctx = E.Context(0)
+    for model, label in cases:
+        ref = {r["mode"]: r for r in rows if r["model"] == model and r["label"] == label}
+        m = L.reference_benchmark_model(model, label)
+        times, e0, dim = [], None, 0
+        for rep in range(3):
+            data = E.build_ed_data(m, ctx=ctx)
+            t0 = time.time()
+            vals, _ = E.ed_scan_at_irrep_matrixfree(data.irrep_list[0].label, data, nev=1, return_vectors=False)
+            times.append(time.time() - t0)
+            e0, dim = float(vals[0]), data.sector_dims[0]
+            data.orbit_catalog.close()
+        t = sum(times[1:]) / 2
+        mf, mx = ref["matrixfree"]["elapsed_s"], ref["matrix"]["elapsed_s"]
+        print(json.dumps({"model": model, "label": label, "sector_dim": dim, "gpu_s": round(t, 5),
+                          "reference_matrixfree_s": mf, "reference_matrix_s": mx,
+                          "speedup_vs_reference_matrixfree": round(mf / t, 1),
+                          "energy": e0, "rel_diff_vs_reference_energy":
+                              abs(e0 - ref["matrixfree"]["energy"]) / abs(ref["matrixfree"]["energy"])}),
+              flush=True)
+
+
 def single_solves(args):
     """One sector solve per precision (plain FP64, then FP32 + FP64 refinement), each
     repeated; prints one JSON line per solve with its Lanczos step counts and time."""
@@ -389,6 +427,8 @@ def main():
     ap.add_argument("--irrep", type=int, default=0, help="momentum sector index for --solve")
     ap.add_argument("--nev", type=int, default=3, help="eigenpairs for --solve")
     ap.add_argument("--solve-repeats", type=int, default=2)
+    ap.add_argument("--reference-cases", action="store_true",
+                    help="only run the reference's single-sector benchmark models and protocol")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -398,6 +438,9 @@ def main():
 
     if args.solve:
         single_solves(args)
+        return
+    if args.reference_cases:
+        reference_cases(args)
         return
 
     if args.copy_calibration:

@@ -385,6 +385,21 @@ def term_arrays(model: Model):
     return i_from, i_to, t, di, dj, V
 
 
+def reference_benchmark_model(model: str, label: str) -> Model:
+    """The models of the reference's benchmark table (``benchmark/benchmark.jl:66-219``,
+    rows of ``benchmark/benchmark_data/benchmark_raw.csv``): ``("Heisenberg", "N=20")``,
+    ``("Haldane_Boson", "3×4")`` (hard-core bosons at one per two cells' worth of sites, the
+    FCI parameters), ``("Hubbard_Fermion", "2×5")`` (U = 8, half filling)."""
+    if model == "Heisenberg":
+        return heisenberg_chain(int(label.split("=")[1]))
+    a, b = (int(x) for x in label.split("×"))
+    if model == "Haldane_Boson":
+        return haldane_honeycomb(a, b)
+    if model == "Hubbard_Fermion":
+        return hubbard_square(a, b, U=8.0)
+    raise KeyError((model, label))
+
+
 # --------------------------------------------------------------------------
 # Named workloads (sizes of the path's benchmark configurations)
 # --------------------------------------------------------------------------

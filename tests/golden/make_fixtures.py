@@ -28,6 +28,7 @@ def main():
                 "n_filled": int(r["n_filled"]), "full_dim": int(r["full_dim"]),
                 "n_orbits": int(r["n_orbits"]), "n_group": int(r["n_group"]),
                 "sector_dim": int(r["sector_dim"]), "mode": r["mode"], "energy": float(r["energy"]),
+                "elapsed_s": float(r["elapsed_s"]),
             })
     nb = json.load(open(os.path.join(REF, "doc", "design.ipynb")))
     outs = {}

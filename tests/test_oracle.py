@@ -82,12 +82,7 @@ CSV_CASES = [
 
 
 def csv_model(model, label):
-    if model == "Heisenberg":
-        return L.heisenberg_chain(int(label.split("=")[1]))
-    a, b = (int(x) for x in label.split("×"))
-    if model == "Haldane_Boson":
-        return L.haldane_honeycomb(a, b)
-    return L.hubbard_square(a, b, U=8.0)
+    return L.reference_benchmark_model(model, label)
 
 
 @pytest.mark.parametrize("model,label", CSV_CASES)
