@@ -146,15 +146,17 @@ int sed_sector_set_shape(sed_sector* sec, int precision, int shape, int xcd_chun
 /* Chosen shape (-1 if not chosen yet), its lanes/batch and XCD run length, the numbers
  * of shapes and of run lengths tried, the autotuner's times in ms (may be NULL;
  * 2 * n_shapes + n_chunks floats: every shape at run length 1, every run length for the
- * fastest of those, every shape again at the fastest run length; 0 if not timed, -1 if
- * the shape does not fit), and the shape's workgroup size (may be NULL). */
+ * fastest of those, the shapes within 20 % of the fastest again at the fastest run
+ * length; 0 if not timed, -1 if the shape does not fit or was not timed again), and the
+ * shape's workgroup size (may be NULL). */
 int sed_sector_shape(const sed_sector* sec, int precision, int* shape, int* lanes, int* batch, int* xcd_chunk,
                      int* n_shapes, int* n_chunks, float* tune_ms, int* workgroup);
 /* diagonalize_block_matrixfree(H_op, n; nev) (:998-1030): lowest nev eigenvalues
  * (ascending) and optionally eigenvectors (sector_dim x nev column-major ComplexF64,
  * may be NULL).  Sectors of dim <= 500 are solved densely, as the reference does;
- * larger ones by thick-restart Lanczos with full reorthogonalisation from a seeded
- * Gaussian start vector, residual tolerance tol (reference: 1e-10), at most maxiter
+ * larger ones by thick-restart Lanczos with partial re-orthogonalisation (the basis is
+ * kept semi-orthogonal; SED_PRO=0 in the environment orthogonalises every step against
+ * the whole basis) from a seeded Gaussian start vector, residual tolerance tol (reference: 1e-10), at most maxiter
  * thick restarts per precision phase (the reference's KrylovKit maxiter = 300 likewise
  * counts restarts of its Krylov basis).  Returns SED_ENOCONV if fewer than nev converge. */
 int sed_solve(sed_sector* sec, int nev, double tol, uint64_t seed, int maxiter, int precision,

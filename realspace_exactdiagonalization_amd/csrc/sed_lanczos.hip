@@ -45,7 +45,9 @@ constexpr int kUpdGrid = 1024;  // fixed grid of the update / norm kernels
 
 // ---- one Lanczos step, step index j = *step on the device ----------------------
 // The step reads v_j through xhat = sqrt(stab) v_j (written by the previous step),
-// w = H v_j, then two classical Gram-Schmidt passes against V[0..j], then
+// w = H v_j, then a classical Gram-Schmidt pass against v_lo..v_j into v_{j+1} (lo set by
+// the partial re-orthogonalisation state: j - 1, or 0 where the whole basis is due), a
+// second pass against V[0..j] in place when pro_decide asks for it, then
 // v_{j+1} = w' / |w'| and its xhat.
 
 // part[i * G + b] = sum over block b's rows of conj(V_i[r]) w[r] for lo <= i <= j, and
