@@ -165,3 +165,23 @@ def test_gpu_spectrum_flow_periodic_and_matches_oracle(oracle_mod, golden):
         for il, (lab, _, w, _) in enumerate(secs):
             assert r.sector_labels[il] == lab
             assert np.abs(r.energies[it, il, :len(w)] - w).max() < 1e-9
+
+
+@pytest.mark.gpu
+def test_gpu_identity_mode_equals_all_sectors(golden):
+    """Without translation symmetry (the reference's :identity mode: one sector holding the
+    whole fixed-N space, 220 states) the flow's lowest energies at every flux equal the
+    merged lowest energies of all momentum sectors (1e-10), and the pump gives the same
+    branch charges as the sector-resolved pump (the notebook's [0.5, 0.5], 1e-8)."""
+    m = fci_model()
+    fl = FLUXES[::4]
+    a = T.flux_spectrum_flow(m, "all", filling_fraction=Fraction(1, 4), nev=4, twisted_phases_over_2pi_list=fl)
+    b = T.flux_spectrum_flow(m, "identity", filling_fraction=Fraction(1, 4), nev=4, twisted_phases_over_2pi_list=fl)
+    assert b.sector_labels == ["identity"] and b.energies.shape == (len(fl), 1, 4)
+    assert np.abs(a.global_energies - b.global_energies).max() < 1e-10
+    assert np.abs(b.energies[:, 0, :] - b.global_energies).max() < 1e-12
+    p = T.flux_charge_pump(m, "identity", filling_fraction=Fraction(1, 4), manifold_size=2,
+                           twisted_phases_over_2pi_list=FLUXES)
+    g = golden["charge_pump_notebook"]["pump"]
+    assert np.allclose(p.pumped_charges, g["pumped_charges"], atol=1e-8)
+    assert abs(p.min_gap - g["min_gap"]) < 1e-11
