@@ -11,7 +11,9 @@ behaviour (exceptions on failure):
     diagonalize_block_matrixfree   :998-1030
     build_ed_data                  :1460-1475
     ed_scan_at_irrep_matrixfree!   :1092-1128   (ed_scan_at_irrep_matrixfree)
-    ed_scan! (flux mode)           :1297-1356   (ed_scan_flux, batched through the server)
+    ed_scan!                       :1297-1356   (ed_scan; flux mode also as ed_scan_flux,
+                                                 batched through the server)
+    ed_scan_at_irrep_matrix!       :1061-1090   (ed_scan_at_irrep_matrix)
 
 Every call runs on the GPU through the HIP library; there is no CPU path here.
 """
@@ -34,7 +36,7 @@ __all__ = [
     "SectorServer", "PREC_FP64", "PREC_FP32", "PREC_FP32_REFINE", "sector_density",
     "density_correlation_matrix", "off_diagonal_long_range_order", "build_ed_Hamiltonian_symmetry_block",
     "expand_sector_state_to_fock_amplitudes", "fock_amplitudes", "position_operator_matrix",
-    "update_orbit_stabilizer_phases",
+    "update_orbit_stabilizer_phases", "ed_scan", "ed_scan_at_irrep_matrix",
 ]
 
 _default_ctx = None
@@ -403,6 +405,69 @@ def ed_scan_at_irrep_matrixfree(irrep_label, ed_data: EDData, nev: int = 5, **so
     vals, vecs = diagonalize_block_matrixfree(H, n, nev=nev, **solve_kw)
     ed_data.ed_scan_res[idx] = (vals, vecs)
     return vals, vecs
+
+
+def ed_scan_at_irrep_matrix(irrep_label, ed_data: EDData, nev: int = 5, **solve_kw):
+    """Matrix-mode per-sector driver (``ed_scan_at_irrep_matrix!``, ``:1061-1090``).  The
+    reference builds the sector block as a sparse matrix and hands it to Arpack (dense
+    ``eigen`` up to dimension 500, ``:978-995``); the block is the operator the matrix-free
+    path applies, so its eigenpairs come from the same GPU solver here (dense up to 500,
+    Lanczos above; Arpack itself is not reproduced).  ``build_ed_Hamiltonian_symmetry_block``
+    returns the block itself."""
+    return ed_scan_at_irrep_matrixfree(irrep_label, ed_data, nev=nev, **solve_kw)
+
+
+def _scan_sectors(ed_data: EDData, nev: int, mode: str, scanned_sectors, **solve_kw) -> EDData:
+    if mode not in ("matrix", "matrixfree"):
+        raise ValueError(f"Unknown ED scan mode: {mode}. Use 'matrix' or 'matrixfree'.")
+    wanted = None if scanned_sectors is None else {tuple(l) for l in scanned_sectors}
+    for idx, ir in enumerate(ed_data.irrep_list):
+        if wanted is not None and tuple(ir.label) not in wanted:
+            continue
+        if idx in ed_data.ed_scan_res:
+            dim = ed_data.sector_dims[idx]
+            if dim == 0 or len(ed_data.ed_scan_res[idx][0]) >= min(nev, dim):
+                continue
+        if mode == "matrixfree":
+            ed_scan_at_irrep_matrixfree(ir.label, ed_data, nev=nev, **solve_kw)
+        else:
+            ed_scan_at_irrep_matrix(ir.label, ed_data, nev=nev, **solve_kw)
+    return ed_data
+
+
+def ed_scan(ed_data: EDData, nev: int = 5, mode: str = "matrix", scanned_sectors=None, *, flux_direction: int = 1,
+            twisted_phases_over_2pi_list: Optional[Sequence[float]] = None, ctx: Optional[Context] = None,
+            **solve_kw):
+    """``ed_scan!`` (``:1297-1356``, sector loop ``_ed_scan_sectors!`` ``:1199-1256``).
+
+    Without a flux list: every momentum sector of ``ed_data`` (or those whose labels are in
+    ``scanned_sectors``) is solved for ``nev`` eigenpairs into ``ed_data.ed_scan_res``;
+    sectors already holding enough pairs are skipped, as a resumed scan skips them.
+    Returns ``ed_data``.  With ``twisted_phases_over_2pi_list``: for each twist theta along
+    ``flux_direction`` (1-based) the model is re-twisted, the flux-aware translation group
+    (or the identity group when ``ed_data`` uses it) and a fresh orbit catalog built, and the
+    sectors scanned; returns the list of per-twist ``EDData`` (the reference writes one
+    checkpoint per twist and returns their paths; checkpoint files are not written here).
+    ``mode`` is ``'matrix'`` (the reference's default) or ``'matrixfree'``; both solve on the
+    GPU (``ed_scan_at_irrep_matrix``)."""
+    if twisted_phases_over_2pi_list is None:
+        return _scan_sectors(ed_data, nev, mode, scanned_sectors, **solve_kw)
+    from .lattice import identity_group
+    model = ed_data.model
+    if not 1 <= flux_direction <= 2:
+        raise ValueError("flux_direction must be 1 or 2")
+    out = []
+    ctx = ctx or ed_data.orbit_catalog.ctx
+    for t in twisted_phases_over_2pi_list:
+        flux = [0.0, 0.0]
+        flux[flux_direction - 1] = float(t)
+        m = twisted(model, flux)
+        m.n_filled = ed_data.n_filled
+        G = identity_group(m.n_site) if ed_data.symmetry_group.name == "identity" else \
+            model_group(m, flux)
+        d = build_ed_data(m, symmetry_group=G, ctx=ctx)
+        out.append(_scan_sectors(d, nev, mode, scanned_sectors, **solve_kw))
+    return out
 
 
 class SectorServer:

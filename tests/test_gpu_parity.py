@@ -706,3 +706,50 @@ def test_device_matvec_all_precisions(ed, ctx, oracle_mod, name):
         assert np.abs(y - ref).max() <= tol * np.abs(ref).max(), (prec, np.abs(y - ref).max())
     gb.close()
     gc.close()
+
+
+def test_ed_scan_sectors_filter_resume_and_flux_mode(ed, ctx, oracle_mod):
+    """ed_scan (ed_scan!, src/symmetry_resolved_ed.jl:1199-1356): every momentum sector of a
+    twisted fermion model solved, each equal to the oracle's lowest eigenvalues (1e-10); a
+    scanned_sectors filter solves only those sectors; a repeated call with the same nev
+    recomputes nothing (resume semantics); matrix mode gives the matrix-free eigenvalues;
+    flux mode returns per-twist data whose eigenvalues equal the batched ed_scan_flux."""
+    base = L.checkerboard(3, 2, n_filled=4, V1=1.0, V2=0.5)
+    th = (0.23, 0.17)
+    m = L.twisted(base, th)
+    G = L.model_group(m, th)
+    data = ed.build_ed_data(m, symmetry_group=G, ctx=ctx)
+    assert ed.ed_scan(data, nev=3, mode="matrixfree") is data
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, True)
+    for idx, ir in enumerate(data.irrep_list):
+        os_ = oracle_mod.Sector(oc, ir.values)
+        vals = data.ed_scan_res[idx][0]
+        assert data.sector_dims[idx] == os_.dim
+        if os_.dim:
+            os_.set_terms(*L.term_arrays(m))
+            ref = oracle_mod.lowest_eigenvalues(os_, min(3, os_.dim))
+            assert np.abs(vals - ref).max() <= 1e-10 * np.abs(ref).max()
+    d2 = ed.build_ed_data(m, symmetry_group=G, ctx=ctx)
+    ed.ed_scan(d2, nev=2, mode="matrix", scanned_sectors=[(0, 0), (2, 1)])
+    labels = {tuple(d2.irrep_list[i].label) for i in d2.ed_scan_res}
+    assert labels == {(0, 0), (2, 1)}
+    kept = {i: d2.ed_scan_res[i] for i in d2.ed_scan_res}
+    ed.ed_scan(d2, nev=2, scanned_sectors=[(0, 0), (2, 1)])
+    assert all(d2.ed_scan_res[i] is kept[i] for i in kept)
+    for i in kept:
+        assert np.abs(kept[i][0] - data.ed_scan_res[i][0][:2]).max() <= 1e-10 * np.abs(kept[i][0]).max()
+    with pytest.raises(ValueError):
+        ed.ed_scan(d2, nev=2, mode="arpack")
+    thetas = [0.0, 0.25, 0.5]
+    b0 = ed.build_ed_data(base, ctx=ctx)
+    per = ed.ed_scan(b0, nev=2, mode="matrixfree", twisted_phases_over_2pi_list=thetas, ctx=ctx)
+    ref = ed.ed_scan_flux(base, [(t, 0.0) for t in thetas], nev=2, ctx=ctx)
+    assert len(per) == len(thetas)
+    for t, d in zip(thetas, per):
+        assert d.model.theta == (t, 0.0)
+        for idx, ir in enumerate(d.irrep_list):
+            if d.sector_dims[idx] == 0:
+                continue
+            a = d.ed_scan_res[idx][0]
+            r = ref[((t, 0.0), tuple(ir.label))]
+            assert np.abs(a - r[:len(a)]).max() <= 1e-10 * np.abs(r).max()
