@@ -102,6 +102,14 @@ __device__ __forceinline__ MvLds<R> stage_mv(const MvConst& c, int block, bool f
 // it takes occupied sites sub, sub+LPR, ... in ascending order.  S points at this
 // thread's rank scratch (k entries, stride SST).  Returns the off-diagonal sum
 // (ax, ay) = sum conj(t') s_JW sign chi(g) xhat[c] and the density sum (dx, dy).
+//
+// Per occupied site a, with mp = m without a (bits q_0 < ... < q_{k-2}), the rank of
+// mp + b for an empty site b that lands at index ib among mp's bits is
+//   sum_{i<ib} C(q_i, i+1) + sum_{i>=ib} C(q_i, i+2) + C(b, ib+1)
+//   = base + S[ib] + C(b, ib+1),  base = sum_i C(q_i, i+2),
+//     S[j] = sum_{i<j} (C(q_i, i+1) - C(q_i, i+2))    (uint32, modulo 2^32),
+// one pass over mp's bits per site.  Offsets into the binomial table are 24-bit
+// products (site < 64, stride < 70).
 template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST>
 __device__ __forceinline__ void row_sums(const MvLds<R>& L, const MvConst& c, const M m, const int sub,
                                          uint32_t* S, const uint32_t* __restrict__ table,
@@ -115,24 +123,19 @@ __device__ __forceinline__ void row_sums(const MvLds<R>& L, const MvConst& c, co
         const int a = ctz_m<M>(rest);
         for (int i = 0; i < LPR && rest; ++i) rest &= rest - 1;
         const M mp = m ^ ((M)1 << a);
+        uint32_t base = 0;
         {
-            uint32_t s0 = 0;
+            uint32_t cur = 0;
+            S[0] = 0;
             M t = mp;
             int j = 0;
             while (t) {
                 const int q = ctz_m<M>(t);
                 t &= t - 1;
-                s0 += L.bin[q * c.bs + j + 2];
-                ++j;
-            }
-            S[0] = s0;
-            t = mp;
-            j = 0;
-            uint32_t cur = s0;
-            while (t) {
-                const int q = ctz_m<M>(t);
-                t &= t - 1;
-                cur += L.bin[q * c.bs + j + 1] - L.bin[q * c.bs + j + 2];
+                const uint32_t* bq = L.bin + __umul24(q, c.bs) + j;
+                const uint32_t c1 = bq[1], c2 = bq[2];
+                base += c2;
+                cur += c1 - c2;
                 ++j;
                 S[j * SST] = cur;
             }
@@ -158,7 +161,7 @@ __device__ __forceinline__ void row_sums(const MvLds<R>& L, const MvConst& c, co
                 const int b = L.tto[tt];
                 const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
                 ok[u] = (t + u < t1) && !((m >> b) & 1);
-                idx[u] = S[ib * SST] + L.bin[b * c.bs + ib + 1];
+                idx[u] = base + S[ib * SST] + L.bin[__umul24(b, c.bs) + ib + 1];
                 pj[u] = FERMI ? ((ib - ia) & 1) : 0;
             }
             uint32_t e[BATCH];
@@ -210,7 +213,8 @@ __device__ __forceinline__ void row_sums_flat(const MvLds<R>& L, const MvConst& 
         while (t) {
             const int q = ctz_m<M>(t);
             t &= t - 1;
-            const uint32_t c0 = L.bin[q * c.bs + j], c1 = L.bin[q * c.bs + j + 1], c2 = L.bin[q * c.bs + j + 2];
+            const uint32_t* bq = L.bin + __umul24(q, c.bs) + j;
+            const uint32_t c0 = bq[0], c1 = bq[1], c2 = bq[2];
             P[j * SST] = pd;
             P[(K1 + j) * SST] = pu;
             Rm += c1;
@@ -239,7 +243,7 @@ __device__ __forceinline__ void row_sums_flat(const MvLds<R>& L, const MvConst& 
             }
             t = L.tstart[a];
             t1 = L.tstart[a + 1];
-            ca = L.bin[a * c.bs + ia + 1];
+            ca = L.bin[__umul24(a, c.bs) + ia + 1];
             if (t < t1) return;
             ia += LPR;
         }
@@ -264,7 +268,7 @@ __device__ __forceinline__ void row_sums_flat(const MvLds<R>& L, const MvConst& 
             const uint32_t mid = (live && a < b) ? P[(ib + 1) * SST] - P[(ia + 1) * SST]
                                                  : P[(K1 + (live ? ia : 0)) * SST] - P[(K1 + ib) * SST];
             ok[u] = live && !((m >> b) & 1);
-            idx[u] = Rm - ca + L.bin[b * c.bs + ib + 1] + mid;
+            idx[u] = Rm - ca + L.bin[__umul24(b, c.bs) + ib + 1] + mid;
             tt[u] = tc;
             pj[u] = FERMI ? ((ib - ia) & 1) : 0;
             if (live) ++t;
