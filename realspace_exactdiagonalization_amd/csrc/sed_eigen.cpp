@@ -78,86 +78,252 @@ static void tridiag_ql(int n, std::vector<double>& d, std::vector<double>& e, st
     }
 }
 
-void hermitian_eigh(int n, std::vector<cd>& A, std::vector<double>& evals, std::vector<cd>* evecs) {
-    // A: column-major n x n Hermitian (only consistency of both triangles assumed).
-    auto a = [&](int i, int j) -> cd& { return A[(size_t)j * n + i]; };
-    std::vector<std::vector<cd>> vs;      // Householder vectors (length n, zeros above k+1)
-    std::vector<double> taus;
-    std::vector<cd> v(n), p(n), q(n);
-    for (int k = 0; k + 2 < n; ++k) {
-        double sigma2 = 0.0;
-        for (int i = k + 1; i < n; ++i) sigma2 += std::norm(a(i, k));
-        const double tail2 = sigma2 - std::norm(a(k + 1, k));
-        if (tail2 <= 0.0) {
-            vs.emplace_back();
-            taus.push_back(0.0);
-            continue;
-        }
-        const double sigma = std::sqrt(sigma2);
-        const cd x0 = a(k + 1, k);
-        const double ax0 = std::abs(x0);
-        const cd ph = ax0 > 0 ? x0 / ax0 : cd(1.0, 0.0);
-        const cd alpha = -ph * sigma;
-        std::fill(v.begin(), v.end(), cd(0.0));
-        for (int i = k + 1; i < n; ++i) v[i] = a(i, k);
-        v[k + 1] -= alpha;
-        const double vnorm2 = 2.0 * sigma * (sigma + ax0);
-        const double tau = 2.0 / vnorm2;
-        // p = tau * B v over the trailing block
-        for (int i = k + 1; i < n; ++i) {
-            cd s = 0.0;
-            for (int j = k + 1; j < n; ++j) s += a(i, j) * v[j];
-            p[i] = tau * s;
-        }
-        cd vhp = 0.0;
-        for (int i = k + 1; i < n; ++i) vhp += std::conj(v[i]) * p[i];
-        const double K = 0.5 * tau * vhp.real();
-        for (int i = k + 1; i < n; ++i) q[i] = p[i] - K * v[i];
-        for (int j = k + 1; j < n; ++j)
-            for (int i = k + 1; i < n; ++i) a(i, j) -= v[i] * std::conj(q[j]) + q[i] * std::conj(v[j]);
-        a(k + 1, k) = alpha;
-        a(k, k + 1) = std::conj(alpha);
-        for (int i = k + 2; i < n; ++i) {
-            a(i, k) = 0.0;
-            a(k, i) = 0.0;
-        }
-        vs.push_back(v);
-        taus.push_back(tau);
+// Eigenvectors of the real symmetric tridiagonal T (diagonal d, off-diagonal e[i]
+// coupling i and i+1) for its eigenvalues w[0..m-1] (ascending, from the QL iteration,
+// accurate to a few ulps of |T|), by inverse iteration: T - sigma is factored once per
+// vector with partial pivoting (U's pivots below eps |T| raised to eps |T|), and the
+// solve is repeated from a fixed pseudo-random start until the growth of the solution
+// shows a residual at rounding level, plus one more solve.  Vectors whose eigenvalues lie
+// within 1e-3 |T| of each other are kept orthogonal by modified Gram-Schmidt after every
+// solve, and equal shifts within such a cluster are separated by 10 eps |T| (the rules
+// of LAPACK's xSTEIN).  z: n x m column-major.
+static void tridiag_inverse_iteration(int n, const std::vector<double>& d, const std::vector<double>& e,
+                                      const std::vector<double>& w, int m, std::vector<double>& z) {
+    z.assign((size_t)n * m, 0.0);
+    double tnorm = 0.0;
+    for (int i = 0; i < n; ++i)
+        tnorm = std::max(tnorm, std::fabs(d[i]) + (i > 0 ? std::fabs(e[i - 1]) : 0.0) +
+                                    (i + 1 < n ? std::fabs(e[i]) : 0.0));
+    if (!(tnorm > 0.0) || n == 1) {   // T = 0 (or 1 x 1): the unit vectors
+        for (int j = 0; j < m; ++j) z[(size_t)j * n + j] = 1.0;
+        return;
     }
-    std::vector<double> d(n), e(std::max(n, 1), 0.0);
-    std::vector<cd> ph(n, cd(1.0, 0.0));
-    for (int i = 0; i < n; ++i) d[i] = a(i, i).real();
-    for (int i = 0; i + 1 < n; ++i) {
-        const cd ei = a(i + 1, i);
+    const double eps = 2.220446049250313e-16;
+    const double pivmin = eps * tnorm;
+    const double ortol = 1e-3 * tnorm;
+    const double pertol = 10.0 * eps * tnorm;
+    const double grow = 1.0 / (1e3 * n * eps * tnorm);   // |solution| that certifies convergence
+    std::vector<double> dd(n), dl(n), du(n), du2(n), x(n);
+    std::vector<char> swap(n);
+    int c0 = 0;           // first vector of the current cluster
+    double sig_prev = 0.0;
+    for (int j = 0; j < m; ++j) {
+        double sigma = w[j];
+        if (j > 0 && w[j] - w[j - 1] > ortol) c0 = j;
+        if (j > c0 && sigma - sig_prev < pertol) sigma = sig_prev + pertol;
+        sig_prev = sigma;
+        // LU of T - sigma with partial pivoting: L unit lower (multipliers dl, row swaps),
+        // U upper with diagonal dd and two super-diagonals du, du2
+        for (int i = 0; i < n; ++i) {
+            dd[i] = d[i] - sigma;
+            dl[i] = du[i] = (i + 1 < n) ? e[i] : 0.0;
+            du2[i] = 0.0;
+        }
+        for (int i = 0; i + 1 < n; ++i) {
+            if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
+                swap[i] = 0;
+                if (dd[i] != 0.0) {
+                    const double f = dl[i] / dd[i];
+                    dl[i] = f;
+                    dd[i + 1] -= f * du[i];
+                } else {
+                    dl[i] = 0.0;
+                }
+            } else {
+                swap[i] = 1;
+                const double f = dd[i] / dl[i];
+                dd[i] = dl[i];
+                dl[i] = f;
+                const double t = du[i];
+                du[i] = dd[i + 1];
+                dd[i + 1] = t - f * dd[i + 1];
+                if (i + 2 < n) {
+                    du2[i] = du[i + 1];
+                    du[i + 1] = -f * du[i + 1];
+                }
+            }
+        }
+        for (int i = 0; i < n; ++i)
+            if (std::fabs(dd[i]) < pivmin) dd[i] = dd[i] < 0.0 ? -pivmin : pivmin;
+        uint64_t st = 0x9e3779b97f4a7c15ull * (uint64_t)(j + 1);
+        double nx = 0.0;
+        for (int i = 0; i < n; ++i) {
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            x[i] = (double)(st >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+            nx += x[i] * x[i];
+        }
+        for (int i = 0; i < n; ++i) x[i] /= std::sqrt(nx);
+        bool conv = false;
+        for (int it = 0; it < 8; ++it) {
+            for (int i = 0; i + 1 < n; ++i) {   // L, with the row swaps
+                if (!swap[i]) {
+                    x[i + 1] -= dl[i] * x[i];
+                } else {
+                    const double t = x[i];
+                    x[i] = x[i + 1];
+                    x[i + 1] = t - dl[i] * x[i];
+                }
+            }
+            for (int i = n - 1; i >= 0; --i) {   // U, rescaling before an overflow
+                double v = x[i];
+                if (i + 1 < n) v -= du[i] * x[i + 1];
+                if (i + 2 < n) v -= du2[i] * x[i + 2];
+                v /= dd[i];
+                if (std::fabs(v) > 1e150) {
+                    const double sc = 1.0 / std::fabs(v);
+                    for (int t = 0; t < n; ++t) x[t] *= sc;
+                    v *= sc;
+                }
+                x[i] = v;
+            }
+            for (int c = c0; c < j; ++c) {
+                const double* zc = &z[(size_t)c * n];
+                double dot = 0.0;
+                for (int i = 0; i < n; ++i) dot += zc[i] * x[i];
+                for (int i = 0; i < n; ++i) x[i] -= dot * zc[i];
+            }
+            double nrm = 0.0;
+            for (int i = 0; i < n; ++i) nrm += x[i] * x[i];
+            nrm = std::sqrt(nrm);
+            if (!(nrm > 0.0)) {   // the start lay in the span of the cluster: a fresh one
+                for (int i = 0; i < n; ++i) {
+                    st = st * 6364136223846793005ull + 1442695040888963407ull;
+                    x[i] = (double)(st >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+                }
+                continue;
+            }
+            for (int i = 0; i < n; ++i) x[i] /= nrm;
+            if (conv) break;
+            conv = nrm >= grow;
+        }
+        std::copy(x.begin(), x.end(), z.begin() + (size_t)j * n);
+    }
+}
+
+// Complex Hermitian n x n (column-major A; only its lower triangle is read): all
+// eigenvalues ascending, and the eigenvectors of the nvec lowest (n x nvec column-major)
+// when evecs is non-null.  Householder reduction to a complex Hermitian tridiagonal form
+// (lower triangle, re/im parts in separate columns so the inner loops run contiguously,
+// complex products written out), a diagonal unitary P making it real, QL for the
+// eigenvalues, inverse iteration on the real tridiagonal matrix for the wanted vectors,
+// and the back-transformation Q P z of those vectors only.
+void hermitian_eigh(int n, std::vector<cd>& A, std::vector<double>& evals, std::vector<cd>* evecs, int nvec) {
+    nvec = std::max(0, std::min(nvec < 0 ? n : nvec, n));
+    const size_t N = (size_t)n;
+    // scaled by a power of two near 1 / max |a_ij| (exact), so that no sum of squares
+    // below can overflow or underflow; the eigenvalues are scaled back at the end
+    double amax = 0.0;
+    for (size_t j = 0; j < N; ++j)
+        for (size_t i = j; i < N; ++i) amax = std::max(amax, std::abs(A[j * N + i]));
+    int ex = 0;
+    if (amax > 0.0 && std::isfinite(amax)) std::frexp(amax, &ex);
+    std::vector<double> ar(N * N), ai(N * N);
+    for (size_t j = 0; j < N; ++j)
+        for (size_t i = j; i < N; ++i) {
+            ar[j * N + i] = std::ldexp(A[j * N + i].real(), -ex);
+            ai[j * N + i] = i == j ? 0.0 : std::ldexp(A[j * N + i].imag(), -ex);
+        }
+    std::vector<double> hr(N * N, 0.0), hi(N * N, 0.0);   // Householder vector k in column k
+    std::vector<double> taus(N, 0.0);
+    std::vector<double> pr(N), pi(N), qr(N), qi(N);
+    for (int k = 0; k + 2 < n; ++k) {
+        const size_t c = (size_t)k * N;
+        double sigma2 = 0.0;
+        for (int i = k + 1; i < n; ++i) sigma2 += ar[c + i] * ar[c + i] + ai[c + i] * ai[c + i];
+        const double x0r = ar[c + k + 1], x0i = ai[c + k + 1];
+        const double ax0 = std::hypot(x0r, x0i);
+        if (sigma2 - ax0 * ax0 <= 0.0) continue;   // column already reduced (tau = 0)
+        const double sigma = std::sqrt(sigma2);
+        const double phr = ax0 > 0 ? x0r / ax0 : 1.0, phi = ax0 > 0 ? x0i / ax0 : 0.0;
+        const double alr = -phr * sigma, ali = -phi * sigma;   // new sub-diagonal entry
+        double* vr = &hr[c];
+        double* vi = &hi[c];
+        for (int i = k + 1; i < n; ++i) {
+            vr[i] = ar[c + i];
+            vi[i] = ai[c + i];
+        }
+        vr[k + 1] -= alr;
+        vi[k + 1] -= ali;
+        const double tau = 1.0 / (sigma * (sigma + ax0));   // 2 / |v|^2
+        // p = tau B v over the trailing block B (lower triangle: a(i,j), i > j, and
+        // a(j,i) = conj(a(i,j)))
+        for (int i = k + 1; i < n; ++i) pr[i] = pi[i] = 0.0;
+        for (int j = k + 1; j < n; ++j) {
+            const double* cr = &ar[(size_t)j * N];
+            const double* ci = &ai[(size_t)j * N];
+            const double vjr = vr[j], vji = vi[j];
+            double sr = cr[j] * vjr, si = cr[j] * vji;
+            for (int i = j + 1; i < n; ++i) {
+                pr[i] += cr[i] * vjr - ci[i] * vji;
+                pi[i] += cr[i] * vji + ci[i] * vjr;
+                sr += cr[i] * vr[i] + ci[i] * vi[i];
+                si += cr[i] * vi[i] - ci[i] * vr[i];
+            }
+            pr[j] += sr;
+            pi[j] += si;
+        }
+        double vhp = 0.0;   // Re v^H p (v^H B v is real)
+        for (int i = k + 1; i < n; ++i) {
+            pr[i] *= tau;
+            pi[i] *= tau;
+            vhp += vr[i] * pr[i] + vi[i] * pi[i];
+        }
+        const double K = 0.5 * tau * vhp;
+        for (int i = k + 1; i < n; ++i) {
+            qr[i] = pr[i] - K * vr[i];
+            qi[i] = pi[i] - K * vi[i];
+        }
+        // B -= v q^H + q v^H on the lower triangle
+        for (int j = k + 1; j < n; ++j) {
+            double* cr = &ar[(size_t)j * N];
+            double* ci = &ai[(size_t)j * N];
+            const double qjr = qr[j], qji = qi[j], vjr = vr[j], vji = vi[j];
+            for (int i = j; i < n; ++i) {
+                cr[i] -= vr[i] * qjr + vi[i] * qji + qr[i] * vjr + qi[i] * vji;
+                ci[i] -= vi[i] * qjr - vr[i] * qji + qi[i] * vjr - qr[i] * vji;
+            }
+            ci[j] = 0.0;
+        }
+        ar[c + k + 1] = alr;
+        ai[c + k + 1] = ali;
+        taus[k] = tau;
+    }
+    // T = P T_r P^H with T_r real: e[i] = |t(i+1,i)|, P = diag(ph)
+    std::vector<double> d(N), e(std::max<size_t>(N, 1), 0.0);
+    std::vector<cd> ph(N, cd(1.0, 0.0));
+    for (size_t i = 0; i < N; ++i) d[i] = ar[i * N + i];
+    for (size_t i = 0; i + 1 < N; ++i) {
+        const cd ei(ar[i * N + i + 1], ai[i * N + i + 1]);
         const double m = std::abs(ei);
         e[i] = m;
-        ph[i + 1] = m > 0 ? ph[i] * ei / m : ph[i];
+        ph[i + 1] = m > 0 ? ph[i] * (ei / m) : ph[i];
     }
+    std::vector<double> w = d, ew = e;
+    tridiag_ql(n, w, ew, nullptr);
+    std::sort(w.begin(), w.end());
+    evals.resize(N);
+    for (size_t i = 0; i < N; ++i) evals[i] = std::ldexp(w[i], ex);
+    if (!evecs || nvec == 0) return;
     std::vector<double> z;
-    if (evecs) {
-        z.assign((size_t)n * n, 0.0);
-        for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;
-    }
-    tridiag_ql(n, d, e, evecs ? &z : nullptr);
-    std::vector<int> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int x, int y) { return d[x] < d[y]; });
-    evals.resize(n);
-    for (int j = 0; j < n; ++j) evals[j] = d[order[j]];
-    if (!evecs) return;
-    // eigenvectors of A: Q P z, Q = H_0 H_1 ... H_{n-3}
+    tridiag_inverse_iteration(n, d, e, w, nvec, z);
     std::vector<cd>& X = *evecs;
-    X.assign((size_t)n * n, cd(0.0));
-    for (int j = 0; j < n; ++j)
-        for (int i = 0; i < n; ++i) X[(size_t)j * n + i] = ph[i] * z[(size_t)i * n + order[j]];
-    for (int k = (int)vs.size() - 1; k >= 0; --k) {
-        if (taus[k] == 0.0) continue;
-        const std::vector<cd>& vk = vs[k];
-        for (int j = 0; j < n; ++j) {
-            cd s = 0.0;
-            for (int i = k + 1; i < n; ++i) s += std::conj(vk[i]) * X[(size_t)j * n + i];
-            s *= taus[k];
-            for (int i = k + 1; i < n; ++i) X[(size_t)j * n + i] -= vk[i] * s;
+    X.assign(N * nvec, cd(0.0));
+    for (int j = 0; j < nvec; ++j) {
+        cd* x = &X[(size_t)j * N];
+        for (size_t i = 0; i < N; ++i) x[i] = ph[i] * z[(size_t)j * N + i];
+        for (int k = n - 3; k >= 0; --k) {   // x <- H_k x, H_k = I - tau_k v_k v_k^H
+            if (taus[k] == 0.0) continue;
+            const double* vr = &hr[(size_t)k * N];
+            const double* vi = &hi[(size_t)k * N];
+            double sr = 0.0, si = 0.0;
+            for (int i = k + 1; i < n; ++i) {
+                sr += vr[i] * x[i].real() + vi[i] * x[i].imag();
+                si += vr[i] * x[i].imag() - vi[i] * x[i].real();
+            }
+            sr *= taus[k];
+            si *= taus[k];
+            for (int i = k + 1; i < n; ++i)
+                x[i] -= cd(vr[i] * sr - vi[i] * si, vr[i] * si + vi[i] * sr);
         }
     }
 }

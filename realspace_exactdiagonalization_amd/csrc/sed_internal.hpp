@@ -238,7 +238,8 @@ inline bool fp32_cluster(const std::vector<double>& theta, int nev) {
 // Real symmetric n x n (row-major, overwritten) -> ascending eigenvalues, eigenvectors in columns.
 void jacobi_eigh(int n, std::vector<double>& a, std::vector<double>& evals, std::vector<double>& evecs);
 void eigh_last_row(int n, std::vector<double>& a, std::vector<double>& evals, std::vector<double>& last);
-// Complex Hermitian n x n (column-major) -> ascending eigenvalues and eigenvectors (columns).
-void hermitian_eigh(int n, std::vector<cd>& a, std::vector<double>& evals, std::vector<cd>* evecs);
+// Complex Hermitian n x n (column-major, lower triangle read) -> all eigenvalues ascending
+// and the eigenvectors of the nvec lowest (n x nvec column-major; nvec < 0: all).
+void hermitian_eigh(int n, std::vector<cd>& a, std::vector<double>& evals, std::vector<cd>* evecs, int nvec = -1);
 
 }  // namespace sed

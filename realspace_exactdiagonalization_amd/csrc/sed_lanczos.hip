@@ -795,15 +795,9 @@ static int solve_dense(sed_sector* sec, int nev, std::vector<double>& evals, std
         }
     for (int64_t r = 0; r < n; ++r) Ac[(size_t)r * n + r] = cd(Ac[(size_t)r * n + r].real(), 0.0);
     std::vector<double> ev;
-    std::vector<cd> X;
-    hermitian_eigh((int)n, Ac, ev, evecs ? &X : nullptr);
     const int k = (int)std::min<int64_t>(nev, n);
+    hermitian_eigh((int)n, Ac, ev, evecs, k);
     evals.assign(ev.begin(), ev.begin() + k);
-    if (evecs) {
-        evecs->assign((size_t)n * k, cd(0.0));
-        for (int j = 0; j < k; ++j)
-            for (int64_t i = 0; i < n; ++i) (*evecs)[(size_t)j * n + i] = X[(size_t)j * n + i];
-    }
     return SED_OK;
 }
 

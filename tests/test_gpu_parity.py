@@ -139,6 +139,53 @@ def test_eigenvalues_vs_oracle_all_sectors(ed, ctx, oracle_mod, name, precision)
             assert np.linalg.norm(r) < 1e-6
 
 
+@pytest.mark.parametrize("twist", [(0.0, 0.0), (0.0, 0.3)])
+def test_dense_path_eigenvectors_in_degenerate_cluster(ed, ctx, oracle_mod, twist):
+    """Dense path (dimension <= 500, src/symmetry_resolved_ed.jl:1004-1017): U = 8
+    Hubbard 2x3 at k = (0, 1), dimension 150, whose levels 2-6 form a five-fold cluster
+    (-2.5606 at zero twist, -2.3580 at twist 0.3 along y).  The six lowest eigenvalues
+    equal the oracle's (1e-12), the returned vectors are orthonormal (1e-12) with
+    residuals |H v - lambda v| at rounding level (1e-11 |H|), and the span of the vectors
+    of each cluster equals the oracle's eigenspace (projector difference 1e-9)."""
+    m = L.twisted(L.reference_benchmark_model("Hubbard_Fermion", "2×3"), twist)
+    G = L.model_group(m, twist)
+    print(f"[dense] hubbard 2x3 k=(0,1) twist={twist}", flush=True)
+    ir = [r for r in L.translation_irreps(G, m.lattice) if tuple(r.label) == (0, 1)][0]
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, True)
+    os_ = oracle_mod.Sector(oc, ir.values)
+    os_.set_terms(*L.term_arrays(m))
+    Hd = os_.dense()
+    Hd = 0.5 * (Hd + Hd.conj().T)
+    w, U = np.linalg.eigh(Hd)
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    gb = ed.build_symmetry_sector_basis(gc, ir)
+    assert gb.dim == os_.dim == 150
+    H, n = ed.hamiltonian_linear_operator(gb, m.bilinear_terms, m.density_terms)
+    nev = 6
+    vals, vecs = ed.diagonalize_block_matrixfree(H, n, nev=nev)
+    hn = np.abs(w).max()
+    assert np.abs(vals - w[:nev]).max() <= 1e-12 * hn
+    assert np.abs(vecs.conj().T @ vecs - np.eye(nev)).max() <= 1e-12
+    for j in range(nev):
+        assert np.linalg.norm(H(vecs[:, j]) - vals[j] * vecs[:, j]) <= 1e-11 * hn
+    # clusters of the oracle's spectrum among the six, each compared as a subspace
+    j = 0
+    while j < nev:
+        c = j
+        while c + 1 < nev and w[c + 1] - w[c] < 1e-8 * hn:
+            c += 1
+        if c + 1 < len(w) and w[c + 1] - w[c] < 1e-8 * hn:   # cluster cut by nev: skip it
+            break
+        P = vecs[:, j:c + 1] @ vecs[:, j:c + 1].conj().T
+        Q = U[:, j:c + 1] @ U[:, j:c + 1].conj().T
+        assert np.abs(P - Q).max() <= 1e-9, (j, c)
+        j = c + 1
+    assert w[5] - w[1] < 1e-8 * hn and w[6] - w[5] > 1e-3   # the five-fold level, whole
+    gb.close()
+    gc.close()
+
+
 GOLDEN = [("Heisenberg", "N=20"), ("Heisenberg", "N=24"), ("Haldane_Boson", "3×4"), ("Haldane_Boson", "2×7"),
           ("Hubbard_Fermion", "2×5"), ("Hubbard_Fermion", "3×4")]
 
