@@ -760,13 +760,13 @@ int sed_sector_one_body(sed_sector* sec, int which, double* rho) {
     tmp.chi = sec->chi;
     tmp.dim = dim;
     tmp.shift = sec->shift;
-    tmp.d_rep = sec->d_rep;
-    tmp.d_stab = sec->d_stab;
-    tmp.d_table = sec->d_table;
-    tmp.d_chi64 = sec->d_chi64;
-    tmp.d_chi32 = sec->d_chi32;
+    tmp.d_rep.borrow(sec->d_rep);
+    tmp.d_stab.borrow(sec->d_stab);
+    tmp.d_table.borrow(sec->d_table);
+    tmp.d_chi64.borrow(sec->d_chi64);
+    tmp.d_chi32.borrow(sec->d_chi32);
     tmp.has_gauge = sec->has_gauge;
-    tmp.d_gauge = sec->d_gauge;
+    tmp.d_gauge.borrow(sec->d_gauge);
     tmp.gauge_f = sec->gauge_f;
     tmp.phases = sec->phases;
     tmp.mv_shape[0] = sec->mv_shape[0] >= 0 ? sec->mv_shape[0] : 1;
@@ -810,24 +810,9 @@ int sed_sector_one_body(sed_sector* sec, int which, double* rho) {
             rho[2 * ((size_t)j * n + i)] = r.real();
             rho[2 * ((size_t)j * n + i) + 1] = -r.imag();
         }
-    // the scratch sector borrowed these tables: hand them back before it goes away
-    tmp.d_rep = DevBuf<uint64_t>();
-    tmp.d_stab = DevBuf<uint8_t>();
-    tmp.d_table = DevBuf<uint32_t>();
-    tmp.d_chi64 = DevBuf<double>();
-    tmp.d_chi32 = DevBuf<float>();
-    tmp.d_gauge = DevBuf<double>();
-    for (auto* b : {&tmp.d_term_start, &tmp.d_dens_start}) b->release();
-    tmp.d_term_to.release();
-    tmp.d_dens_j.release();
-    tmp.d_term_coef64.release();
-    tmp.d_dens_v64.release();
-    tmp.d_term_coef32.release();
-    tmp.d_dens_v32.release();
-    y.release();
-    xh.release();
-    part.release();
-    x.release();
+    // the scratch sector's own term lists and the work vectors are freed on return (the
+    // borrowed tables are not); nothing is in flight after launch_cdot's synchronization
+    SED_HIP(hipStreamSynchronize(s));
     return rc;
 }
 
@@ -943,22 +928,9 @@ int sed_sector_destroy(sed_sector* sec) {
     // the synchronous workspace's captured step references this sector's tables
     Workspace* ws = sec->cat->ctx->ws;
     if (ws && ws->graph_key.sec == sec) ws->drop_graph();
-    for (auto* b : {&sec->d_rep}) b->release();
-    sec->d_stab.release();
-    sec->d_table.release();
-    sec->d_term_start.release();
-    sec->d_dens_start.release();
-    sec->d_term_to.release();
-    sec->d_dens_j.release();
-    sec->d_term_coef64.release();
-    sec->d_dens_v64.release();
-    sec->d_chi64.release();
-    sec->d_term_coef32.release();
-    sec->d_dens_v32.release();
-    sec->d_chi32.release();
-    sec->d_gauge.release();
+    (void)hipStreamSynchronize(sec->cat->ctx->stream);
     sed_catalog* cat = sec->cat;
-    delete sec;
+    delete sec;   // frees the sector's device tables
     if (cat->n_sectors_alive == 0 && cat->released) free_catalog(cat);
     return SED_OK;
 }

@@ -32,11 +32,36 @@ int fail(int code, const std::string& msg);
         if (rc_ != SED_OK) return rc_; \
     } while (0)
 
-// RAII-free device buffer: freed explicitly by the owning handle.
+// Device buffer.  It owns its allocation (freed by release() or when it goes out of
+// scope; move-only), or, after borrow(), points at another buffer's allocation without
+// ever freeing it.  Code that frees a buffer a kernel may still be reading synchronizes
+// its stream first.
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool owned = true;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), owned(o.owned) {
+        o.p = nullptr;
+        o.n = 0;
+        o.owned = true;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            owned = o.owned;
+            o.p = nullptr;
+            o.n = 0;
+            o.owned = true;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     int alloc(size_t count) {
         release();
         if (count == 0) count = 1;
@@ -49,10 +74,17 @@ struct DevBuf {
         n = count;
         return SED_OK;
     }
+    void borrow(const DevBuf& o) {
+        release();
+        p = o.p;
+        n = o.n;
+        owned = false;
+    }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        owned = true;
     }
     size_t bytes() const { return n * sizeof(T); }
 };

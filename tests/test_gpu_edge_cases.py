@@ -179,3 +179,52 @@ def test_error_codes_and_recovery(ed, ctx):
     gb.close()
     gc.close()
     assert np.array_equal(valid_solve(), ref)
+
+
+def _device_free_bytes():
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value
+
+
+def test_calls_release_their_device_buffers(ed, ctx):
+    """Regression: the synchronous entry points free the device buffers they allocate per
+    call (matvec in both precisions, solve with eigenvectors, site densities, pair
+    densities, Fock expansion).  Heisenberg N=24, k=0 (dimension 112,720): ten rounds of
+    those calls after two warm-up rounds leave the device's free memory within 24 MiB
+    (one round used to keep about 11 MB)."""
+    m = L.heisenberg_chain(24)
+    G = L.model_group(m)
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    gb = ed.build_symmetry_sector_basis(gc, L.translation_irreps(G, m.lattice)[0])
+    assert gb.dim == 112720
+    H, n = ed.hamiltonian_linear_operator(gb, m.bilinear_terms, m.density_terms)
+    x = np.random.default_rng(5).standard_normal(n) + 0.25j
+
+    def one_round():
+        H.precision = ed.PREC_FP64
+        y64 = H(x)
+        H.precision = ed.PREC_FP32
+        y32 = H(x)
+        H.precision = ed.PREC_FP64
+        assert np.abs(y32 - y64).max() <= 1e-5 * np.abs(y64).max()
+        vals, vecs = ed.diagonalize_block_matrixfree(H, n, nev=2)
+        d = ed.sector_density(H, 0)
+        assert abs(d.sum() - m.n_filled) < 1e-9
+        ed.density_correlation_matrix(H, 0)
+        ed.fock_amplitudes(vecs[:, 0], gb)
+        return vals
+
+    ref = one_round()
+    one_round()
+    free0 = _device_free_bytes()
+    for _ in range(10):
+        assert np.abs(one_round() - ref).max() < 1e-10
+    free1 = _device_free_bytes()
+    print(f"[memory] free before {free0 / 2**20:.1f} MiB, after {free1 / 2**20:.1f} MiB", flush=True)
+    assert free0 - free1 < 24 * 2**20
+    gb.close()
+    gc.close()
