@@ -216,7 +216,10 @@ int sed_server_submit(sed_server* srv, sed_sector* sec, int nev, double tol, uin
  * between calls.  n_done receives the number of requests finished so far. */
 int sed_server_run(sed_server* srv, int max_steps, int64_t* n_done);
 /* Result of a request: status 0 = done, 1 = pending, 2 = running, <0 = failed (error
- * code); evals (nev doubles) and stats are filled when done. */
+ * code); evals (nev doubles) and stats are filled when done.  A result can be read any
+ * number of times: the server keeps every request's record (its eigenvalues, stats and
+ * timestamps, about 200 bytes + 8 per eigenvalue) until it is destroyed, so a
+ * long-running service recreates its server from time to time. */
 int sed_server_result(sed_server* srv, int64_t request_id, int* status, double* evals, sed_solve_stats* stats,
                       double* latency_s);
 int sed_server_destroy(sed_server* srv);
