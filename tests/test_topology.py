@@ -185,3 +185,44 @@ def test_gpu_identity_mode_equals_all_sectors(golden):
     g = golden["charge_pump_notebook"]["pump"]
     assert np.allclose(p.pumped_charges, g["pumped_charges"], atol=1e-8)
     assert abs(p.min_gap - g["min_gap"]) < 1e-11
+
+
+def fermionic_fci_model():
+    """test/fermionic_fci.jl:12-143: checkerboard 3x4 (24 sites), 8 fermions, the
+    Sun-Gu-Katsura-Das Sarma parameters with the bands flipped (every hopping amplitude
+    negated), V1 = 2, V2 = 1, V3 = 0."""
+    s2 = math.sqrt(2.0)
+    return L.checkerboard(3, 4, t=-1.0, phi_over_2pi=1 / 8, tp1=-1 / (2 + s2), tp2=1 / (2 + s2),
+                          tpp=-1 / (2 + 2 * s2), V1=2.0, V2=1.0, V3=0.0, n_filled=8)
+
+
+@pytest.mark.gpu
+def test_gpu_fermionic_fci_charge_pump(oracle_mod):
+    """test_fermionic_fci_charge_pump (test/fermionic_fci.jl:311-376): 9 fluxes 0:1/8:1
+    along x (the first direction of size divisible by 3), polarization along y, the FCI
+    sectors (0,0), (1,0), (2,0) with one state each.  Every branch pumps |Q| = 2/3 within
+    the reference's own tolerance 0.10 (it stores no output for this run), the three
+    branches sum to an integer, and at zero flux the three manifold energies equal the
+    oracle's (ARPACK on the oracle's sectors, 1e-9): -5.8509 twice and -5.7829."""
+    m = fermionic_fci_model()
+    fluxes = [i / 8 for i in range(9)]
+    labels = [(0, 0), (1, 0), (2, 0)]
+    r = T.flux_charge_pump(m, labels, filling_fraction=Fraction(1, 3), flux_direction=1,
+                           polarization_direction=2, twisted_phases_over_2pi_list=fluxes, nev_per_sector=1)
+    print("[fermionic pump] charges", r.pumped_charges, "min gap", r.min_gap, flush=True)
+    q = np.asarray(r.pumped_charges)
+    assert q.shape == (3,) and np.all(np.isfinite(r.gaps))
+    assert np.all(np.abs(np.abs(q) - 2 / 3) <= 0.10)
+    assert abs(q.sum() - round(q.sum())) <= 0.10
+    G = L.model_group(m)
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, True)
+    sel = sorted((s["sector"], s["level"]) for s in r.selected_states[0])
+    assert sorted(lab for lab, _ in sel) == labels
+    for ir in L.translation_irreps(G, m.lattice):
+        if tuple(ir.label) not in labels:
+            continue
+        os_ = oracle_mod.Sector(oc, ir.values)
+        os_.set_terms(*L.term_arrays(m))
+        ref = oracle_mod.lowest_eigenvalues(os_, 1)
+        got = r.energies[0, r.sector_labels.index(tuple(ir.label)), 0]
+        assert abs(got - ref[0]) <= 1e-9 * abs(ref[0]), (ir.label, got, ref)
