@@ -582,6 +582,32 @@ def test_density_correlation_and_one_body_match_oracle(ed, ctx, oracle_mod, name
     gc.close()
 
 
+@pytest.mark.parametrize("tpp", [-0.8, -0.58, -0.2])
+def test_odlro_demo(ed, ctx, oracle_mod, tpp):
+    """The ODLRO demo of test/bosonic_fci.jl:277-335 on the GPU: Haldane bosons 2x3, 3
+    bosons, k = (0,0) ground state; the one-body density matrix from the GPU, transformed
+    to rho(k) on the reference's 61 x 61 grid, passes the demo's assertions (finite,
+    >= -1e-10, peak at Gamma for the superfluid at t'' = -0.2) and equals the map of the
+    oracle's one-body matrix of the same eigenvector (1e-12)."""
+    from tests.test_oracle import odlro_checks, odlro_k_map
+    m = L.haldane_honeycomb(2, 3, tpp=tpp, n_filled=3)
+    G = L.model_group(m)
+    ir = L.translation_irreps(G, m.lattice)[0]
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    gb = ed.build_symmetry_sector_basis(gc, ir)
+    H, n = ed.hamiltonian_linear_operator(gb, m.bilinear_terms, m.density_terms)
+    vals, vecs = ed.diagonalize_block_matrixfree(H, n, nev=1)
+    k, M = odlro_k_map(m.lattice, ed.off_diagonal_long_range_order(H, 0))
+    odlro_checks(k, M, tpp)
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, False)
+    os_ = oracle_mod.Sector(oc, ir.values, basis=(gb.representative_mask_list, gb.stabilizer_order_list))
+    _, M_ref = odlro_k_map(m.lattice, os_.one_body(vecs[:, 0]))
+    assert np.abs(M - M_ref).max() < 1e-12
+    gb.close()
+    gc.close()
+
+
 @pytest.mark.parametrize("name", ["hald2x3", "hub2x3", "checker3x2_f_flux", "checker3x2_b_flux", "hald3x2_flux"])
 def test_matrix_mode_block_matches_oracle(ed, ctx, oracle_mod, golden, name):
     """Matrix mode (build_ed_Hamiltonian_symmetry_block) from the GPU tables against the

@@ -419,3 +419,48 @@ def test_flux_refresh_equals_rebuild(oracle_mod, case):
     assert np.abs(oa[3] - ob[3]).max() < 1e-14
     if th0 != th1:
         assert np.abs(a[2] - oracle_mod.Catalog(base.n_site, base.n_filled, G0, ferm).tables()[2]).max() > 1e-3
+
+
+# ODLRO demo of test/bosonic_fci.jl:277-335: Haldane bosons 2x3, half a boson per cell
+# (3 bosons), k = (0,0) ground state at t'' = -0.8 (superfluid near M), -0.58 (FCI) and
+# -0.2 (superfluid near Gamma)
+ODLRO_CASES = (-0.8, -0.58, -0.2)
+
+
+def odlro_k_map(lattice, rho, k_resolution=61):
+    """rho(k) = (1/N) sum_ij conj(e^{ik.r_i}) rho_ij e^{ik.r_j} on the k_resolution^2 grid
+    over [-1.5 pi, 1.5 pi]^2 (observables/off_diagonal_long_range_order.jl:179-198), with
+    the site positions r = (cell + sub_crys) . (a1, a2), a1 = (1, 0), a2 = (1/2, sqrt 3/2)
+    (test/bosonic_fci.jl:33-37)."""
+    a1 = np.array([1.0, 0.0])
+    a2 = np.array([0.5, math.sqrt(3.0) / 2])
+    r = np.array([(x + lattice.sub_crys[s - 1][0]) * a1 + (y + lattice.sub_crys[s - 1][1]) * a2
+                  for (x, y), s in lattice.site_list])
+    k = np.linspace(-1.5 * math.pi, 1.5 * math.pi, k_resolution)
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    U = np.exp(1j * (kx[..., None] * r[:, 0] + ky[..., None] * r[:, 1]))
+    return k, np.einsum("abi,ij,abj->ab", U.conj(), rho, U).real / len(r)
+
+
+def odlro_checks(k, rho_map, tpp):
+    """The reference demo's assertions: finite, rho(k) >= -1e-10 (the one-body density
+    matrix is positive semidefinite), and at t'' = -0.2 the maximum at Gamma within one
+    grid step."""
+    assert rho_map.shape == (len(k), len(k)) and np.all(np.isfinite(rho_map))
+    assert rho_map.min() >= -1e-10
+    if tpp == -0.2:
+        ix, iy = np.unravel_index(np.argmax(rho_map), rho_map.shape)
+        step = k[1] - k[0]
+        assert abs(k[ix]) <= step and abs(k[iy]) <= step
+
+
+@pytest.mark.parametrize("tpp", ODLRO_CASES)
+def test_oracle_odlro_demo(oracle_mod, tpp):
+    m = L.haldane_honeycomb(2, 3, tpp=tpp, n_filled=3)
+    G = L.model_group(m)
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, False)
+    sec = oracle_mod.Sector(oc, L.translation_irreps(G, m.lattice)[0].values)
+    sec.set_terms(*L.term_arrays(m))
+    w, v = np.linalg.eigh(sec.dense())
+    k, rho_map = odlro_k_map(m.lattice, sec.one_body(v[:, 0]))
+    odlro_checks(k, rho_map, tpp)
