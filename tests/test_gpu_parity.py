@@ -608,6 +608,34 @@ def test_odlro_demo(ed, ctx, oracle_mod, tpp):
     gc.close()
 
 
+@pytest.mark.parametrize("case", range(4))
+def test_structure_factor_demo(ed, ctx, oracle_mod, case):
+    """The static structure factor of test/bosonic_fci.jl:359-397 (Haldane bosons 2x3 at
+    the three t'' of the demo) and of twisted checkerboard fermions: S(q) from the GPU's
+    connected density correlations on the reference's 61 x 61 grid is finite and
+    non-negative (a covariance), vanishes at Gamma (fixed particle number) to 1e-12, and
+    equals the map of the oracle's correlations of the same eigenvector (1e-12)."""
+    from tests.test_oracle import SQ_CASES, k_map, sq_checks, sq_model
+    m, th = sq_model(*SQ_CASES[case])
+    G = L.model_group(m, th)
+    ir = L.translation_irreps(G, m.lattice)[0]
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    gb = ed.build_symmetry_sector_basis(gc, ir)
+    H, n = ed.hamiltonian_linear_operator(gb, m.bilinear_terms, m.density_terms)
+    vals, vecs = ed.diagonalize_block_matrixfree(H, n, nev=1)
+    Cc, d = ed.density_correlation_matrix(H, 0)
+    k, S = k_map(m.lattice, Cc)
+    sq_checks(k, S)
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, m.statistics == L.FERMIONIC)
+    os_ = oracle_mod.Sector(oc, ir.values, basis=(gb.representative_mask_list, gb.stabilizer_order_list))
+    C_ref, d_ref = os_.density_correlation(vecs[:, 0])
+    _, S_ref = k_map(m.lattice, C_ref - np.outer(d_ref, d_ref))
+    assert np.abs(S - S_ref).max() < 1e-12
+    gb.close()
+    gc.close()
+
+
 @pytest.mark.parametrize("name", ["hald2x3", "hub2x3", "checker3x2_f_flux", "checker3x2_b_flux", "hald3x2_flux"])
 def test_matrix_mode_block_matches_oracle(ed, ctx, oracle_mod, golden, name):
     """Matrix mode (build_ed_Hamiltonian_symmetry_block) from the GPU tables against the

@@ -427,19 +427,30 @@ def test_flux_refresh_equals_rebuild(oracle_mod, case):
 ODLRO_CASES = (-0.8, -0.58, -0.2)
 
 
-def odlro_k_map(lattice, rho, k_resolution=61):
-    """rho(k) = (1/N) sum_ij conj(e^{ik.r_i}) rho_ij e^{ik.r_j} on the k_resolution^2 grid
-    over [-1.5 pi, 1.5 pi]^2 (observables/off_diagonal_long_range_order.jl:179-198), with
-    the site positions r = (cell + sub_crys) . (a1, a2), a1 = (1, 0), a2 = (1/2, sqrt 3/2)
-    (test/bosonic_fci.jl:33-37)."""
+def site_positions(lattice):
+    """Cartesian site positions r = (cell + sub_crys) . (a1, a2): a1 = (1, 0) and
+    a2 = (1/2, sqrt 3/2) on the honeycomb (test/bosonic_fci.jl:33-37), a2 = (0, 1) on the
+    checkerboard (test/fermionic_fci.jl:43-49)."""
     a1 = np.array([1.0, 0.0])
-    a2 = np.array([0.5, math.sqrt(3.0) / 2])
-    r = np.array([(x + lattice.sub_crys[s - 1][0]) * a1 + (y + lattice.sub_crys[s - 1][1]) * a2
-                  for (x, y), s in lattice.site_list])
+    a2 = np.array([0.5, math.sqrt(3.0) / 2]) if lattice.name == "Haldane_Honeycomb" else np.array([0.0, 1.0])
+    return np.array([(x + lattice.sub_crys[s - 1][0]) * a1 + (y + lattice.sub_crys[s - 1][1]) * a2
+                     for (x, y), s in lattice.site_list])
+
+
+def k_map(lattice, M, k_resolution=61):
+    """(1/N) Re sum_ij conj(e^{ik.r_i}) M_ij e^{ik.r_j} on the k_resolution^2 grid over
+    [-1.5 pi, 1.5 pi]^2 (observables/off_diagonal_long_range_order.jl:179-198,
+    observables/static_structure_factor.jl:572-582)."""
+    r = site_positions(lattice)
     k = np.linspace(-1.5 * math.pi, 1.5 * math.pi, k_resolution)
     kx, ky = np.meshgrid(k, k, indexing="ij")
     U = np.exp(1j * (kx[..., None] * r[:, 0] + ky[..., None] * r[:, 1]))
-    return k, np.einsum("abi,ij,abj->ab", U.conj(), rho, U).real / len(r)
+    return k, np.einsum("abi,ij,abj->ab", U.conj(), M, U).real / len(r)
+
+
+def odlro_k_map(lattice, rho, k_resolution=61):
+    """rho(k) of a one-body density matrix rho_ij = <a_i^dag a_j>."""
+    return k_map(lattice, rho, k_resolution)
 
 
 def odlro_checks(k, rho_map, tpp):
@@ -464,3 +475,34 @@ def test_oracle_odlro_demo(oracle_mod, tpp):
     w, v = np.linalg.eigh(sec.dense())
     k, rho_map = odlro_k_map(m.lattice, sec.one_body(v[:, 0]))
     odlro_checks(k, rho_map, tpp)
+
+
+# Static structure factor of test/bosonic_fci.jl:359-397 (and of any sector state):
+# S(q) from the connected <n_i n_j> - <n_i><n_j>, a covariance matrix, so S(q) >= 0, and
+# S(Gamma) = (<N^2> - <N>^2) / N = 0 in a fixed-particle-number sector
+SQ_CASES = [("hald", -0.8), ("hald", -0.58), ("hald", -0.2), ("checker_flux", None)]
+
+
+def sq_model(kind, tpp):
+    if kind == "hald":
+        return L.haldane_honeycomb(2, 3, tpp=tpp, n_filled=3), (0.0, 0.0)
+    return L.twisted(L.checkerboard(3, 2, n_filled=4, V1=1.0, V2=0.5), (0.23, 0.17)), (0.23, 0.17)
+
+
+def sq_checks(k, S):
+    assert np.all(np.isfinite(S)) and S.min() >= -1e-12
+    i0 = int(np.argmin(np.abs(k)))
+    assert k[i0] == 0.0 and abs(S[i0, i0]) < 1e-12
+
+
+@pytest.mark.parametrize("kind,tpp", SQ_CASES)
+def test_oracle_structure_factor_demo(oracle_mod, kind, tpp):
+    m, th = sq_model(kind, tpp)
+    G = L.model_group(m, th)
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, m.statistics == L.FERMIONIC)
+    sec = oracle_mod.Sector(oc, L.translation_irreps(G, m.lattice)[0].values)
+    sec.set_terms(*L.term_arrays(m))
+    w, v = np.linalg.eigh(sec.dense())
+    C, d = sec.density_correlation(v[:, 0])
+    k, S = k_map(m.lattice, C - np.outer(d, d))
+    sq_checks(k, S)
