@@ -619,19 +619,24 @@ int sed_matvec_benchmark(sed_sector* sec, int precision, int n_calls, double* ms
     if (fp64) SED_TRY(launch_matvec64(sec, (const double*)x.p, (double*)y.p, (double*)xh.p, s));
     else SED_TRY(launch_matvec32(sec, (const float*)x.p, (float*)y.p, (float*)xh.p, s));
     for (int i = 0; i < 2; ++i) SED_TRY(launch_matvec_xhat(sec, xh.p, y.p, fp64, s));
-    hipEvent_t e0, e1;
-    SED_HIP(hipEventCreate(&e0));
-    SED_HIP(hipEventCreate(&e1));
-    SED_HIP(hipEventRecord(e0, s));
-    for (int i = 0; i < n_calls; ++i) SED_TRY(launch_matvec_xhat(sec, xh.p, y.p, fp64, s));
-    SED_HIP(hipEventRecord(e1, s));
-    SED_HIP(hipEventSynchronize(e1));
+    hipEvent_t ev[2] = {nullptr, nullptr};
     float ms = 0;
-    SED_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *ms_per_call = ms / n_calls;
-    return SED_OK;
+    auto timed = [&]() -> int {
+        SED_HIP(hipEventCreate(&ev[0]));
+        SED_HIP(hipEventCreate(&ev[1]));
+        SED_HIP(hipEventRecord(ev[0], s));
+        for (int i = 0; i < n_calls; ++i) SED_TRY(launch_matvec_xhat(sec, xh.p, y.p, fp64, s));
+        SED_HIP(hipEventRecord(ev[1], s));
+        SED_HIP(hipEventSynchronize(ev[1]));
+        SED_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        return SED_OK;
+    };
+    const int rc = timed();
+    (void)hipStreamSynchronize(s);   // the buffers are freed on return
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc == SED_OK) *ms_per_call = ms / n_calls;
+    return rc;
 }
 
 int sed_sector_set_shape(sed_sector* sec, int precision, int shape, int xcd_chunk) {
