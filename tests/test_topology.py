@@ -226,3 +226,61 @@ def test_gpu_fermionic_fci_charge_pump(oracle_mod):
         ref = oracle_mod.lowest_eigenvalues(os_, 1)
         got = r.energies[0, r.sector_labels.index(tuple(ir.label)), 0]
         assert abs(got - ref[0]) <= 1e-9 * abs(ref[0]), (ir.label, got, ref)
+
+
+def flux_regression_model(statistics):
+    """The model of test/flux_regression.jl:6-8: phase_exploration/src/model.jl:1-71 at
+    x = 0 (config.jl:9-19, t'' = x / (2 + 2 sqrt 2) = 0), 3x2 cells, 2 particles
+    (config.jl:117-121), for either statistics."""
+    s2 = math.sqrt(2.0)
+    return L.checkerboard(3, 2, t=-1.0, phi_over_2pi=1 / 8, tp1=-1.4 / (2 + s2), tp2=1.4 / (2 + s2), tpp=0.0,
+                          V1=2.0, V2=0.45, V3=0.2, lam=2.8, n_filled=2, statistics=statistics)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("statistics", [L.BOSONIC, L.FERMIONIC])
+def test_gpu_flux_regression(statistics):
+    """test/flux_regression.jl on the GPU path.  At fluxes (0,0), (0.23,0.17), (1,0)
+    (:5-49): every momentum sector's block from matrix mode is Hermitian (1e-11), the
+    matrix-free product with x = 1:n + i equals the block times x (1e-10), and the union
+    of all sector spectra equals the identity-group spectrum (1e-10).  Spectral flow over
+    [(0,0)] and :identity at fluxes 0, 0.23, 1 with nev = 4 (:50-62): equal global
+    energies (1e-10), flux 0 equal to flux 1 (1e-10), and a second run equal to the first
+    (1e-12)."""
+    from realspace_exactdiagonalization_amd import ed
+    base = flux_regression_model(statistics)
+    print(f"[flux regression] statistics={statistics}", flush=True)
+    ctx = ed.default_context()
+    for flux in [(0.0, 0.0), (0.23, 0.17), (1.0, 0.0)]:
+        m = L.twisted(base, flux)
+        spectra = []
+        for G in (L.model_group(m, flux), L.identity_group(m.n_site)):
+            gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                                 particle_statistics=m.statistics, ctx=ctx)
+            vals_all = []
+            for ir in L.translation_irreps(G, m.lattice):
+                gb = ed.build_symmetry_sector_basis(gc, ir)
+                if gb.dim == 0:
+                    gb.close()
+                    continue
+                Hs = ed.build_ed_Hamiltonian_symmetry_block(gb, m.bilinear_terms, m.density_terms).toarray()
+                assert np.abs(Hs - Hs.conj().T).max() < 1e-11
+                H, n = ed.hamiltonian_linear_operator(gb, m.bilinear_terms, m.density_terms)
+                x = np.arange(1, n + 1) + 1j
+                assert np.abs(H(x) - Hs @ x).max() < 1e-10
+                vals, _ = ed.diagonalize_block_matrixfree(H, n, nev=n, return_vectors=False)
+                assert len(vals) == n
+                vals_all.extend(vals)
+                gb.close()
+            gc.close()
+            spectra.append(np.sort(vals_all))
+        assert len(spectra[0]) == len(spectra[1]) == math.comb(12, 2)
+        assert np.abs(spectra[0] - spectra[1]).max() < 1e-10
+    fluxes = [0.0, 0.23, 1.0]
+    a = T.flux_spectrum_flow(base, [(0, 0)], nev=4, twisted_phases_over_2pi_list=fluxes)
+    b = T.flux_spectrum_flow(base, "identity", nev=4, twisted_phases_over_2pi_list=fluxes)
+    assert np.all(np.isfinite(b.energies))
+    assert np.abs(a.global_energies - b.global_energies).max() < 1e-10
+    assert np.abs(a.global_energies[0] - a.global_energies[-1]).max() < 1e-10
+    again = T.flux_spectrum_flow(base, [(0, 0)], nev=4, twisted_phases_over_2pi_list=fluxes)
+    assert np.abs(again.global_energies - a.global_energies).max() < 1e-12
