@@ -20,3 +20,21 @@ extern "C" void host_hermitian_eigh(int n, const double* a, double* evals, doubl
             evecs[2 * i + 1] = X[i].imag();
         }
 }
+
+// Real symmetric n x n (row-major): eigenvalues ascending and eigenvectors (columns of the
+// row-major evecs), or the last row of the eigenvector matrix only.
+extern "C" void host_jacobi_eigh(int n, const double* a, double* evals, double* evecs) {
+    std::vector<double> A(a, a + (size_t)n * n), e, X;
+    sed::jacobi_eigh(n, A, e, X);
+    for (int i = 0; i < n; ++i) evals[i] = e[i];
+    for (size_t i = 0; i < X.size(); ++i) evecs[i] = X[i];
+}
+
+extern "C" void host_eigh_last_row(int n, const double* a, double* evals, double* last) {
+    std::vector<double> A(a, a + (size_t)n * n), e, l;
+    sed::eigh_last_row(n, A, e, l);
+    for (int i = 0; i < n; ++i) {
+        evals[i] = e[i];
+        last[i] = l[i];
+    }
+}

@@ -6,7 +6,11 @@ matrices, exactly degenerate spectra (U diag U^H with triple levels, identical d
 blocks: the tridiagonal form splits), diagonal, identity, zero and rank-one matrices,
 entries near 1e-200 and 1e200, a graded matrix and Wilkinson's W21+ pattern (pairs equal
 to many digits): eigenvalues within 1e-13 |A|, residuals |A x - lambda x| within 1e-12 |A|,
-eigenvectors orthonormal within 1e-12."""
+eigenvectors orthonormal within 1e-12.  The Lanczos loop's real symmetric solvers
+(projected matrix, order <= 64: jacobi_eigh and eigh_last_row, the latter carrying only
+the last row of the eigenvector matrix for the residual estimates) are checked on
+tridiagonal and thick-restart arrowhead matrices the same way, the last row against
+LAPACK's eigenvectors up to sign."""
 import ctypes
 import os
 import shutil
@@ -30,7 +34,17 @@ def heig(tmp_path_factory):
     assert r.returncode == 0, r.stderr
     lib = ctypes.CDLL(out)
 
-    def run(A, nvec):
+    def run(A, nvec=None):
+        if nvec is None:   # real symmetric: (evals, evecs) and (evals, last row)
+            n = A.shape[0]
+            Ar = np.ascontiguousarray(A, dtype=np.float64)
+            e1, X = np.zeros(n), np.zeros((n, n))
+            e2, last = np.zeros(n), np.zeros(n)
+            lib.host_jacobi_eigh(ctypes.c_int(n), Ar.ctypes.data_as(ctypes.c_void_p), e1.ctypes.data_as(ctypes.c_void_p),
+                                 X.ctypes.data_as(ctypes.c_void_p))
+            lib.host_eigh_last_row(ctypes.c_int(n), Ar.ctypes.data_as(ctypes.c_void_p),
+                                   e2.ctypes.data_as(ctypes.c_void_p), last.ctypes.data_as(ctypes.c_void_p))
+            return e1, X, e2, last
         n = A.shape[0]
         Af = np.asfortranarray(A.astype(np.complex128))
         ev = np.zeros(n)
@@ -99,3 +113,37 @@ def test_special_matrices(heig):
     n = 21   # Wilkinson W21+: eigenvalue pairs equal to ~14 digits
     W = np.diag(np.abs(np.arange(n) - 10.0)) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
     check(heig, W.astype(complex), 21)
+
+
+def arrowhead(kk, m):
+    """The projected matrix after a thick restart: Ritz values on the diagonal of the
+    first kk rows, their couplings in row kk, tridiagonal beyond."""
+    T = np.zeros((m, m))
+    T[np.arange(kk), np.arange(kk)] = np.sort(rng.standard_normal(kk))
+    T[kk, :kk] = T[:kk, kk] = 1e-3 * rng.standard_normal(kk)
+    for i in range(kk, m):
+        T[i, i] = rng.standard_normal()
+        if i + 1 < m:
+            T[i, i + 1] = T[i + 1, i] = abs(rng.standard_normal())
+    return T
+
+
+@pytest.mark.parametrize("kind,m", [("tridiagonal", 40), ("tridiagonal", 64), ("arrowhead", 40), ("arrowhead", 64),
+                                    ("degenerate", 30)])
+def test_projected_matrix_solvers(heig, kind, m):
+    if kind == "tridiagonal":
+        T = np.diag(rng.standard_normal(m)) + np.diag(np.abs(rng.standard_normal(m - 1)), 1)
+        T = T + np.triu(T, 1).T
+    elif kind == "arrowhead":
+        T = arrowhead(m // 3, m)
+    else:   # kept Ritz values that coincide, no coupling: T splits
+        T = np.diag(np.repeat([-2.0, -1.0, 0.5], m // 3))
+    e1, X, e2, last = heig(T)
+    w, U = np.linalg.eigh(T)
+    tn = np.abs(w).max()
+    assert np.abs(e1 - w).max() <= 1e-13 * tn and np.abs(e2 - w).max() <= 1e-13 * tn
+    assert np.abs(T @ X - X * e1).max() <= 1e-12 * tn
+    assert np.abs(X.T @ X - np.eye(m)).max() <= 1e-12
+    assert np.abs(last - X[-1]).max() <= 1e-12
+    if kind != "degenerate":   # eigenvectors unique up to sign
+        assert np.abs(np.abs(last) - np.abs(U[-1])).max() <= 1e-10
