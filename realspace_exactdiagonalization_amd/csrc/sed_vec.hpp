@@ -57,7 +57,7 @@ struct ProState {
 };
 
 // Start of a Krylov process (step 0): first passes span the basis until step 2.
-__device__ __forceinline__ void pro_start(ProState* pro, bool fp64, bool enabled) {
+__host__ __device__ inline void pro_start(ProState* pro, bool fp64, bool enabled) {
     pro->lo = 0;
     pro->force = 0;
     pro->kk = 0;
@@ -81,8 +81,8 @@ __device__ __forceinline__ void pro_start(ProState* pro, bool fp64, bool enabled
 // pass (or a first pass over an orthonormal basis, steps 0 and 1) omega_{j+1,.} is reset
 // to eps; the two vectors a local pass orthogonalises against get eps.
 // nrm2 = |w'|^2 after the first pass, nw = |w|^2 before it, aj = Re h_j of the first pass.
-// One thread.
-__device__ inline bool pro_decide(ProState* pro, int j, double nrm2, double nw, double aj,
+// One thread (host-callable, so the decision logic can be driven from a CPU model).
+__host__ __device__ inline bool pro_decide(ProState* pro, int j, double nrm2, double nw, double aj,
                                   const double* alpha, const double* beta) {
     bool need = !(nrm2 > kReorthEta2 * nw);
     if (pro->force > 0) {
@@ -136,7 +136,7 @@ __device__ inline bool pro_decide(ProState* pro, int j, double nrm2, double nw, 
 
 // End of step j (alpha_j = a, beta_j = b final): the next step's first pass covers
 // v_j, v_{j+1} (the whole basis when partial re-orthogonalisation is off).  One thread.
-__device__ __forceinline__ void pro_advance(ProState* pro, int j, double a, double b, const double* beta) {
+__host__ __device__ inline void pro_advance(ProState* pro, int j, double a, double b, const double* beta) {
     pro->anorm = fmax(pro->anorm, fabs(a) + b + (j > 0 ? beta[j - 1] : 0.0));
     pro->lo = pro->enabled ? j : 0;
 }
@@ -144,7 +144,7 @@ __device__ __forceinline__ void pro_advance(ProState* pro, int j, double a, doub
 // After a thick restart to kk kept Ritz vectors (values theta, arrowhead couplings arrow):
 // the diagonal of T, no coupling among them, step kk's first pass over the whole basis
 // (the arrowhead), and the second pass at steps kk and kk + 1.  One thread.
-__device__ __forceinline__ void pro_restart(ProState* pro, int kk, const double* theta, const double* arrow,
+__host__ __device__ inline void pro_restart(ProState* pro, int kk, const double* theta, const double* arrow,
                                             double* alpha, double* beta) {
     for (int i = 0; i < kk; ++i) {
         alpha[i] = theta[i];
