@@ -154,11 +154,16 @@ int sed_sector_shape(const sed_sector* sec, int precision, int* shape, int* lane
 /* diagonalize_block_matrixfree(H_op, n; nev) (:998-1030): lowest nev eigenvalues
  * (ascending) and optionally eigenvectors (sector_dim x nev column-major ComplexF64,
  * may be NULL).  Sectors of dim <= 500 are solved densely, as the reference does;
- * larger ones by thick-restart Lanczos with partial re-orthogonalisation (the basis is
- * kept semi-orthogonal; SED_PRO=0 in the environment orthogonalises every step against
- * the whole basis) from a seeded Gaussian start vector, residual tolerance tol (reference: 1e-10), at most maxiter
- * thick restarts per precision phase (the reference's KrylovKit maxiter = 300 likewise
- * counts restarts of its Krylov basis).  Returns SED_ENOCONV if fewer than nev converge. */
+ * larger ones by thick-restart Lanczos from a seeded Gaussian start vector, residual
+ * tolerance tol (reference: 1e-10), at most maxiter thick restarts per precision phase
+ * (the reference's KrylovKit maxiter = 300 likewise counts restarts of its Krylov basis).
+ * For nev <= 5 the basis is kept semi-orthogonal by partial re-orthogonalisation and the
+ * FP64 phase checks the true residuals of its pairs (one matvec each), repeating the
+ * solve with whole-basis passes if one exceeds tol; larger nev orthogonalises every step
+ * against the whole basis (SED_PRO=0 in the environment does so for every nev,
+ * SED_PRO_MAX_NEV moves the limit).  stats->residual_max is the largest checked residual
+ * (the Lanczos estimate where none was checked).  Returns SED_ENOCONV if fewer than nev
+ * converge. */
 int sed_solve(sed_sector* sec, int nev, double tol, uint64_t seed, int maxiter, int precision,
               double* evals, double* evecs, sed_solve_stats* stats);
 /* Site densities <n_i> of eigenvector `which` (0-based) of the last sed_solve on this

@@ -321,6 +321,27 @@ __global__ void __launch_bounds__(256) k_convert(const Tin* __restrict__ a, Tout
         b[r] = from_d2<Tout>(to_d2(a[r]));
 }
 
+// Partial sums of |w - theta v|^2 (the true residual of a Ritz pair, w = H v).
+template <typename T>
+__global__ void __launch_bounds__(256) k_resid_part(const T* __restrict__ w, const T* __restrict__ v, double theta,
+                                                    int64_t n, double* __restrict__ part) {
+    __shared__ double red[256 / kWave];
+    double acc = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const double2 a = to_d2(w[r]), b = to_d2(v[r]);
+        const double x = a.x - theta * b.x, y = a.y - theta * b.y;
+        acc += x * x + y * y;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sm = 0.0;
+        for (int k = 0; k < 256 / kWave; ++k) sm += red[k];
+        part[blockIdx.x] = sm;
+    }
+}
+
 // sum of nv vectors (complex double out)
 template <typename T>
 __global__ void __launch_bounds__(256) k_sum_vectors(const T* __restrict__ V, int64_t ld, int nv,
@@ -436,6 +457,13 @@ static bool pro_enabled() {
     return !(e && e[0] == '0');
 }
 
+// Largest nev solved with partial re-orthogonalisation in the per-sector path
+// (kProMaxNev; SED_PRO_MAX_NEV overrides it, e.g. to exercise the residual check).
+static int pro_max_nev() {
+    const char* e = std::getenv("SED_PRO_MAX_NEV");
+    return e && *e ? std::atoi(e) : kProMaxNev;
+}
+
 static bool graphs_enabled() {
     const char* e = std::getenv("SED_LANCZOS_GRAPH");
     return !(e && e[0] == '0');
@@ -457,7 +485,7 @@ class ThickLanczos {
         const size_t nd = 4 * (size_t)(m_ + 2) + 2 * (size_t)m_ + 1;
         const size_t npro = (sizeof(ProState) + sizeof(double) - 1) / sizeof(double);
         SED_TRY(grow(ws_.state, nd + 4 + npro));
-        SED_TRY(ws_.ensure_host(nd + 8));
+        SED_TRY(ws_.ensure_host(std::max<size_t>(nd + 8, kUpdGrid)));
         V_ = reinterpret_cast<T*>(VecTraits<T>::V(ws_));
         w_ = reinterpret_cast<T*>(VecTraits<T>::W(ws_));
         xh_ = reinterpret_cast<T*>(VecTraits<T>::X(ws_));
@@ -595,6 +623,28 @@ class ThickLanczos {
         SED_TRY(read_full_steps(out.full_steps));
         return SED_OK;
     }
+
+    // True residuals |H x_i - theta_i x_i| of the nk Ritz pairs in V[0..nk) (FP64 vectors:
+    // one FP64 matvec each, through the step's w and x-hat buffers).
+    int true_residuals(int nk, const std::vector<double>& theta, std::vector<double>& res) {
+        res.assign(nk, 0.0);
+        for (int i = 0; i < nk; ++i) {
+            SED_TRY(launch_matvec64(sec_, reinterpret_cast<const double*>(vec(i)), reinterpret_cast<double*>(w_),
+                                    reinterpret_cast<double*>(xh_), st_));
+            hipLaunchKernelGGL(k_resid_part<T>, dim3(kUpdGrid), dim3(256), 0, st_, w_, vec(i), theta[i], (int64_t)n_,
+                               normpart_);
+            SED_HIP(hipGetLastError());
+            SED_HIP(hipMemcpyAsync(ws_.hcoef, normpart_, kUpdGrid * sizeof(double), hipMemcpyDeviceToHost, st_));
+            SED_HIP(hipStreamSynchronize(st_));
+            double sm = 0.0;
+            for (int b = 0; b < kUpdGrid; ++b) sm += ws_.hcoef[b];
+            res[i] = std::sqrt(sm);
+        }
+        return SED_OK;
+    }
+    bool pro_on() const { return pro_on_; }
+    // every step orthogonalised against the whole basis (before init())
+    void disable_pro() { pro_on_ = false; }
 
     int nritz_ = 0;
     T* V_ = nullptr;
@@ -801,6 +851,54 @@ static int solve_dense(sed_sector* sec, int nev, std::vector<double>& evals, std
     return SED_OK;
 }
 
+// The convergence estimates beta |s_i| hold for the Lanczos relation of a semi-orthogonal
+// basis.  With many kept Ritz vectors the relation of a partially re-orthogonalised run can
+// be off by more than tol (true residuals up to 1e-7 measured on a spin-degenerate Hubbard
+// sector at nev = 20; within tol at nev <= 5), so the FP64 phase's pairs are checked with
+// one FP64 matvec each.  If one exceeds tol, the FP64 solve is repeated with a pass over the
+// whole basis at every step, started from the sum of the Ritz vectors (from the seeded random
+// vector when the wanted values are degenerate).  On return o holds the final pairs (true
+// residuals when they were checked and passed) and vecs the device address of their vectors.
+static int verify_fp64(sed_sector* sec, int m, int nev, double tol, uint64_t seed, int maxiter, Workspace& ws,
+                       ThickLanczos<double2>& L, SolveOut& o, sed_solve_stats& st, double2*& vecs) {
+    vecs = L.V_;
+    if (!L.pro_on() || o.converged < nev) return SED_OK;
+    const int nk = (int)o.evals.size();
+    std::vector<double> res;
+    SED_TRY(L.true_residuals(nk, o.evals, res));
+    st.matvecs += nk;
+    double rmax = 0.0;
+    for (double r : res) rmax = std::max(rmax, r);
+    if (rmax <= tol) {
+        o.resid = res;
+        return SED_OK;
+    }
+    const int64_t n = sec->dim;
+    hipStream_t s = ws.stream;
+    ThickLanczos<double2> F(sec, m, ws);
+    F.disable_pro();
+    if (fp32_cluster(o.evals, nev)) {
+        SED_TRY(F.init());
+        SED_TRY(F.start_random(seed));
+    } else {
+        SED_TRY(grow(ws.v0, (size_t)2 * n));
+        double2* v0 = reinterpret_cast<double2*>(ws.v0.p);
+        hipLaunchKernelGGL(k_sum_vectors<double2>, dim3(grid_for(n)), dim3(256), 0, s, L.V_, (int64_t)n, L.nritz_, v0,
+                           (int64_t)n);
+        SED_HIP(hipGetLastError());
+        SED_TRY(F.init());
+        SED_TRY(F.start_from(v0));
+    }
+    // the first run's work is counted here, the repeated run's by the caller
+    st.iters_fp64 += o.iters;
+    st.matvecs += o.matvecs;
+    st.restarts += o.restarts;
+    st.full_orth_steps += o.full_steps;
+    SED_TRY(F.run(nev, tol, maxiter, o));
+    vecs = F.V_;
+    return SED_OK;
+}
+
 static void fill_resid(sed_solve_stats& st, const SolveOut& o) {
     for (double r : o.resid) st.residual_max = std::max(st.residual_max, r);
 }
@@ -839,6 +937,7 @@ int solve_sector(sed_sector* sec, int nev, double tol, uint64_t seed, int maxite
     SolveOut o;
     if (precision == SED_PREC_FP32 || precision == SED_PREC_FP32_REFINE) {
         ThickLanczos<float2> L32(sec, m, ws);
+        if (nev > pro_max_nev()) L32.disable_pro();
         SED_TRY(L32.init());
         SED_TRY(L32.start_random(seed));
         // FP32 vectors resolve residuals down to ~1e-6 |H|; stop well above that floor.
@@ -865,6 +964,7 @@ int solve_sector(sed_sector* sec, int nev, double tol, uint64_t seed, int maxite
         // FP64 refinement restarted from the sum of the FP32 Ritz vectors (from the seeded
         // random vector when the wanted FP32 Ritz values are degenerate)
         ThickLanczos<double2> L64(sec, m, ws);
+        if (nev > pro_max_nev()) L64.disable_pro();
         if (fp32_cluster(o.evals, nev)) {
             SED_TRY(L64.init());
             SED_TRY(L64.start_random(seed));
@@ -878,20 +978,25 @@ int solve_sector(sed_sector* sec, int nev, double tol, uint64_t seed, int maxite
             SED_TRY(L64.start_from(v0));
         }
         SED_TRY(L64.run(nev, tol, maxiter, o));
-        st.iters_fp64 = o.iters;
+        double2* vecs = nullptr;
+        SED_TRY(verify_fp64(sec, m, nev, tol, seed, maxiter, ws, L64, o, st, vecs));
+        st.iters_fp64 += o.iters;
         if (out_vecs) {
             SED_TRY(out_vecs->alloc((size_t)n * o.evals.size()));
-            SED_HIP(hipMemcpyAsync(out_vecs->p, L64.V_, (size_t)n * o.evals.size() * 16, hipMemcpyDeviceToDevice, s));
+            SED_HIP(hipMemcpyAsync(out_vecs->p, vecs, (size_t)n * o.evals.size() * 16, hipMemcpyDeviceToDevice, s));
         }
     } else {
         ThickLanczos<double2> L64(sec, m, ws);
+        if (nev > pro_max_nev()) L64.disable_pro();
         SED_TRY(L64.init());
         SED_TRY(L64.start_random(seed));
         SED_TRY(L64.run(nev, tol, maxiter, o));
-        st.iters_fp64 = o.iters;
+        double2* vecs = nullptr;
+        SED_TRY(verify_fp64(sec, m, nev, tol, seed, maxiter, ws, L64, o, st, vecs));
+        st.iters_fp64 += o.iters;
         if (out_vecs) {
             SED_TRY(out_vecs->alloc((size_t)n * o.evals.size()));
-            SED_HIP(hipMemcpyAsync(out_vecs->p, L64.V_, (size_t)n * o.evals.size() * 16, hipMemcpyDeviceToDevice, s));
+            SED_HIP(hipMemcpyAsync(out_vecs->p, vecs, (size_t)n * o.evals.size() * 16, hipMemcpyDeviceToDevice, s));
         }
     }
     st.matvecs += o.matvecs;

@@ -36,6 +36,9 @@ constexpr int kPackBlock = 1024;
 constexpr int kPackCheck = 16;            // steps between host checks (flux scan, FP32+refine
                                           // solves/s: 8 -> 845, 16 -> 891, 32 -> 859)
 constexpr int64_t kPackMaxDim = 1 << 15;   // one CU per sector: larger sectors go to the lanes
+// The packed kernel always uses partial re-orthogonalisation: requests for more eigenpairs
+// than it covers go to the lanes.
+constexpr int kPackMaxNev = kProMaxNev;
 constexpr int kPackDots = 4;              // basis vectors per sweep of the dot products
 constexpr int kPackBatch = 4;             // hops whose table reads (then gathers) issue together
                                           // (flux scan: 2 -> 1097, 4 -> 1162, 8 -> 912 solves/s)
@@ -392,7 +395,8 @@ static size_t pack_lds(const sed_sector* sec) {
 }
 
 bool pack_eligible(const sed_sector* sec, int nev) {
-    if (sec->dim <= 500 || sec->dim > kPackMaxDim || nev > krylov_dim(sec->dim, nev) - 2) return false;
+    if (sec->dim <= 500 || sec->dim > kPackMaxDim || nev > kPackMaxNev || nev > krylov_dim(sec->dim, nev) - 2)
+        return false;
     // static LDS of the kernel is below 4 KiB
     return pack_lds(sec) + 4096 <= (size_t)sec->cat->ctx->max_lds;
 }

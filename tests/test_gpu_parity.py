@@ -491,6 +491,55 @@ PRO_CASES = {
 }
 
 
+@pytest.mark.parametrize("mode", ["default", "pro_forced", "full"])
+def test_degenerate_spectrum_nev20(ed, ctx, oracle_mod, monkeypatch, mode):
+    """Spin-degenerate spectrum with the largest basis: U = 8 Hubbard 2x4 at twist
+    (0.21, 0.37), k = 0 (dimension 1638), nev = 20.  "default": above the partial
+    re-orthogonalisation limit (nev <= 5) every step is orthogonalised against the whole
+    basis; "pro_forced": partial re-orthogonalisation forced on (SED_PRO_MAX_NEV=64), whose
+    pairs miss tol here, so the FP64 phase's residual check must catch that and repeat the
+    solve with whole-basis passes; "full": SED_PRO=0.  Single-vector Lanczos may miss
+    copies of a degenerate level (as the reference's KrylovKit can), so the bar is what
+    must hold regardless: every returned value is an eigenvalue of the oracle's matrix
+    (1e-9), none is returned more often than its multiplicity (no ghost copies), and the
+    returned vectors are orthonormal (1e-8) with residuals |H x - lambda x| <= 1e-9."""
+    if mode == "full":
+        monkeypatch.setenv("SED_PRO", "0")
+    if mode == "pro_forced":
+        monkeypatch.setenv("SED_PRO_MAX_NEV", "64")
+    th = (0.21, 0.37)
+    m = L.twisted(L.hubbard_square(2, 4, U=8.0), th)
+    G = L.model_group(m, th)
+    ir = L.translation_irreps(G, m.lattice)[0]
+    oc = oracle_mod.Catalog(m.n_site, m.n_filled, G, True)
+    os_ = oracle_mod.Sector(oc, ir.values)
+    os_.set_terms(*L.term_arrays(m))
+    Hd = os_.dense()
+    w = np.linalg.eigvalsh(0.5 * (Hd + Hd.conj().T))
+    hn = np.abs(w).max()
+    gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
+                                         particle_statistics=m.statistics, ctx=ctx)
+    gb = ed.build_symmetry_sector_basis(gc, ir)
+    H, n = ed.hamiltonian_linear_operator(gb, m.bilinear_terms, m.density_terms)
+    assert n == 1638
+    st = {}
+    vals, vecs = ed.diagonalize_block_matrixfree(H, n, nev=20, stats=st)
+    print(f"[degenerate nev20] {mode}: steps {st['iters_fp64']} full {st['full_orth_steps']} "
+          f"restarts {st['restarts']} matvecs {st['matvecs']} residual {st['residual_max']:.1e}", flush=True)
+    for lam in vals:
+        assert np.abs(w - lam).min() <= 1e-9 * hn
+        mult = int(np.sum(np.abs(w - lam) <= 1e-8 * hn))
+        assert int(np.sum(np.abs(vals - lam) <= 1e-8 * hn)) <= mult
+    assert np.abs(vecs.conj().T @ vecs - np.eye(20)).max() <= 1e-8
+    for j in range(20):
+        assert np.linalg.norm(H(vecs[:, j]) - vals[j] * vecs[:, j]) <= 1e-9
+    assert st["residual_max"] <= 1e-10
+    if mode == "pro_forced":   # the check ran (20 extra matvecs) and the repeat was needed
+        assert st["matvecs"] > st["iters_fp64"]
+    gb.close()
+    gc.close()
+
+
 @pytest.mark.parametrize("name,reorth", [("checker3x4_f", "mfma"), ("checker3x4_f", None),
                                          ("hald3x4_flux_nev8", None), ("hald3x4_flux_nev20", "mfma"),
                                          ("xxz20", None)])
@@ -499,7 +548,8 @@ def test_partial_reorthogonalisation_matches_full(ed, ctx, oracle_mod, monkeypat
     orthogonalised against the whole basis) against a pass over the whole basis at every
     step (SED_PRO=0): the same eigenvalues to 1e-11 relative in FP64 and FP32+refine, both
     within 1e-10 of the oracle, the same number of Lanczos steps within 10 %, at most half
-    as many steps orthogonalised against the whole basis, and bitwise reproducible runs.
+    as many steps orthogonalised against the whole basis for nev <= 5 (larger nev always
+    orthogonalises against the whole basis), and bitwise reproducible runs.
     Sectors with several thick restarts: checkerboard 3x4 fermions (dim 61,333; also with
     the matrix-core kernels forced, SED_REORTH=mfma, which large sectors use by default),
     the twisted Haldane 3x4 bosons with nev = 8 (dim ~11,200) and with nev = 20 (the
@@ -541,7 +591,11 @@ def test_partial_reorthogonalisation_matches_full(ed, ctx, oracle_mod, monkeypat
         steps_a = sa["iters_fp32"] + sa["iters_fp64"]
         steps_b = sb["iters_fp32"] + sb["iters_fp64"]
         assert abs(steps_a - steps_b) <= 0.1 * steps_b
-        assert sb["full_orth_steps"] == steps_b and sa["full_orth_steps"] <= 0.5 * steps_a
+        assert sb["full_orth_steps"] == steps_b
+        if nev <= 5:
+            assert sa["full_orth_steps"] <= 0.5 * steps_a
+        else:   # more wanted pairs than partial re-orthogonalisation is used for
+            assert sa["full_orth_steps"] == steps_a
     gb.close()
     gc.close()
 
