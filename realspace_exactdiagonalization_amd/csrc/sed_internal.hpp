@@ -221,7 +221,8 @@ int launch_catalog_amp(const sed_catalog* cat, double* d_amp);  // reference rep
 int launch_sector(sed_sector* sec, const double* chi_dev);
 int launch_hop_counts(sed_sector* sec, int64_t* gated, int64_t* in_sector);
 
-// y = H' x in the sector basis (gauge-transformed, see DESIGN); x, y device vectors.
+// y = H' x in the gauge-free sector basis (H = D^-1 H' D, D = diag of the representatives'
+// twist gauge factors); x, y device vectors.
 // prescale: xhat = sqrt(stab) * x.   FP64: double2, FP32: float2.
 int launch_matvec64(sed_sector* sec, const double* x, double* y, double* xhat, hipStream_t s);
 int launch_matvec32(sed_sector* sec, const float* x, float* y, float* xhat, hipStream_t s);
