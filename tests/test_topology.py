@@ -284,3 +284,45 @@ def test_gpu_flux_regression(statistics):
     assert np.abs(a.global_energies[0] - a.global_energies[-1]).max() < 1e-10
     again = T.flux_spectrum_flow(base, [(0, 0)], nev=4, twisted_phases_over_2pi_list=fluxes)
     assert np.abs(again.global_energies - a.global_energies).max() < 1e-12
+
+
+def phase_protocol_model():
+    """The diagnostic of test/phase_flux_protocol.jl:8: phase_exploration/src/model.jl:1-71
+    with config.jl:9-19's parameters at t'' = -0.3, checkerboard 3x3 (18 sites), 3 fermions
+    (config.jl:117-121)."""
+    s2 = math.sqrt(2.0)
+    return L.checkerboard(3, 3, t=-1.0, phi_over_2pi=1 / 8, tp1=-1.4 / (2 + s2), tp2=1.4 / (2 + s2), tpp=-0.3,
+                          V1=2.0, V2=0.45, V3=0.2, lam=2.8, n_filled=3)
+
+
+def test_oracle_phase_protocol_pump(oracle_mod):
+    """test/phase_flux_protocol.jl:9-31 (pump of the three-state manifold over 17 fluxes
+    0:1/16:1 along x, polarization along y): every manifold gap above 1e-9 and the branch
+    charges summing to 1 within 1e-9, on the oracle pipeline."""
+    r = TO.charge_pump(oracle_mod, phase_protocol_model(), FLUXES, 3)
+    assert np.all(np.asarray(r.gaps) > 1e-9)
+    assert abs(r.pumped_charges.sum() - 1.0) < 1e-9
+
+
+@pytest.mark.gpu
+def test_gpu_phase_protocol_pump(oracle_mod):
+    """The same protocol through the product driver (diagnostics.jl:276-283: :all sectors,
+    manifold_size 3): gaps above 1e-9, position singular values above 1e-10, three selected
+    states at every flux and branch charges summing to 1 within 1e-9 (the reference test's
+    assertions), with the per-flux gaps (1e-11), projected-position eigenvalues (1e-10, as
+    sets) and branch charges (1e-8) of the oracle pipeline."""
+    m = phase_protocol_model()
+    r = T.flux_charge_pump(m, "all", manifold_size=3, flux_direction=1, polarization_direction=2,
+                           twisted_phases_over_2pi_list=FLUXES)
+    print("[phase protocol pump] charges", r.pumped_charges, "sum", r.pumped_charges.sum(), flush=True)
+    assert np.all(r.gaps > 1e-9) and np.all(r.min_position_singular_values > 1e-10)
+    assert all(len(s) == 3 for s in r.selected_states)
+    assert abs(r.pumped_charges.sum() - 1.0) < 1e-9
+    o = TO.charge_pump(oracle_mod, m, FLUXES, 3)
+    assert np.abs(r.gaps - np.asarray(o.gaps)).max() < 1e-11
+    # as sets per flux: at fluxes 0 and 1 one eigenvalue lies on the negative real axis,
+    # where ordering by phase depends on the sign of a 1e-16 imaginary part
+    for a, b in zip(r.polarization_eigenvalues, o.polarization_eigenvalues):
+        d = np.abs(a[:, None] - b[None, :])
+        assert d.min(axis=1).max() < 1e-10 and d.min(axis=0).max() < 1e-10
+    assert np.abs(np.sort(r.pumped_charges) - np.sort(o.pumped_charges)).max() < 1e-8
