@@ -94,6 +94,12 @@ def lib():
         L.oc_fock_expand.restype = C.c_int64
         L.oc_fock_expand.argtypes = [C.c_void_p, _f64p, C.c_double, _f64p]
         L.oc_position_operator.argtypes = [C.c_void_p, C.c_void_p, _f64p, _i64p, _f64p]
+        L.oc_check_sector_basis.restype = C.c_int64
+        L.oc_check_sector_basis.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _f64p, C.c_int, _f64p, C.c_double,
+                                            C.c_int64, _u64p, _i64p, C.c_int, C.POINTER(C.c_int64)]
+        L.oc_sample_rows.argtypes = [C.c_int, C.c_int, _i32p, _f64p, C.c_int, _f64p, C.c_int64, _u64p, _i64p,
+                                     C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _i32p, _f64p, _f64p,
+                                     C.c_int64, _i64p, _f64p, C.c_int]
         _lib = L
     return _lib
 
@@ -363,3 +369,46 @@ def lowest_eigenvalues(sec: Sector, nev: int, tol: float = 1e-13) -> np.ndarray:
     v0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     vals = eigsh(H, k=nev, which="SA", tol=tol, v0=v0, return_eigenvectors=False, ncv=max(2 * nev + 20, 40))
     return np.sort(vals.real)
+
+
+def _threads(nthreads: int) -> int:
+    return nthreads if nthreads > 0 else max(1, min(16, os.cpu_count() or 1))
+
+
+def check_sector_basis(group, n_filled: int, fermionic: bool, chi, reps, stab, atol: float = 1e-12,
+                       nthreads: int = 0):
+    """Check a listed sector basis against the definition of ``build_symmetry_sector_basis``
+    (``src/symmetry_resolved_ed.jl:534-556``) without building the catalog: every entry a
+    canonical representative with ``n_filled`` bits (``:442-453``), strictly ascending,
+    its stabilizer order, and its stabilizer phases equal to the characters
+    (``_is_orbit_compatible`` ``:534-542``).  Returns (number of failing entries, first
+    failing index or -1)."""
+    reps = np.ascontiguousarray(reps, dtype=np.uint64)
+    stab = np.ascontiguousarray(stab, dtype=np.int64)
+    perm = np.ascontiguousarray(group.perm, dtype=np.int32)
+    ph = _c128(group.phases)
+    chi = _c128(chi)
+    first = C.c_int64(-1)
+    bad = lib().oc_check_sector_basis(group.n_site, n_filled, group.order, perm, ph, 1 if fermionic else 0, chi,
+                                      atol, len(reps), reps, stab, _threads(nthreads), C.byref(first))
+    return int(bad), int(first.value)
+
+
+def sample_rows(model, group, chi, reps, stab, x, rows, nthreads: int = 0) -> np.ndarray:
+    """``(H x)[rows]`` (0-based rows) of the sector with the listed basis, from the entries
+    ``build_matrixfree_projection_table`` (``src/symmetry_resolved_ed.jl:680-728``) would
+    write, found from each row without rep tables or Hermiticity (``oc_sample_rows``)."""
+    from realspace_exactdiagonalization_amd.lattice import term_arrays, FERMIONIC
+    a, b, t, di, dj, V = term_arrays(model)
+    reps = np.ascontiguousarray(reps, dtype=np.uint64)
+    stab = np.ascontiguousarray(stab, dtype=np.int64)
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    assert len(x) == len(reps) == len(stab) and (len(rows) == 0 or (rows.min() >= 0 and rows.max() < len(reps)))
+    perm = np.ascontiguousarray(group.perm, dtype=np.int32)
+    y = np.zeros(len(rows), dtype=np.complex128)
+    lib().oc_sample_rows(group.n_site, group.order, perm, _c128(group.phases),
+                         1 if model.statistics == FERMIONIC else 0, _c128(chi), len(reps), reps, stab,
+                         len(a), a, b, _c128(t), len(di), di, dj, _c128(V), x.view(np.float64), len(rows), rows,
+                         y.view(np.float64), _threads(nthreads))
+    return y

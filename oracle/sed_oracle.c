@@ -802,3 +802,148 @@ int oc_catalog_update_phases(oc_catalog* c, const double* phases) {
         }
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Sectors too large for the catalog tables: a listed basis checked    */
+/* against its definition, and sampled rows of H x                     */
+/* ------------------------------------------------------------------ */
+
+/* A group view over caller arrays (no inverse table: only apply_op is used). */
+static oc_group group_view(int n_site, int n_group, const int32_t* perm, const double* phases, int fermionic) {
+    oc_group G;
+    G.n_site = n_site; G.n_group = n_group; G.fermionic = fermionic;
+    G.perm = (int32_t*)perm; G.phases = (cplx*)phases; G.inv = NULL;
+    return G;
+}
+
+/* The sector basis of build_symmetry_sector_basis (:544-556) is the ascending list of
+ * catalog representatives (masks with n_filled bits that no group element maps lower,
+ * :442-453) whose stabilizer phases equal the characters (_is_orbit_compatible,
+ * :534-542), each with its stabilizer order.  Checks a listed basis entry by entry
+ * against that definition: strictly ascending, n_filled bits, canonical, stabilizer
+ * order, compatibility.  Together with a length equal to the character-formula
+ * dimension this fixes the list.  Returns the number of entries that fail (and the
+ * first one, 0-based, in *first_bad; -1 if none). */
+int64_t oc_check_sector_basis(int n_site, int n_filled, int n_group, const int32_t* perm, const double* phases,
+                              int fermionic, const double* chi_, double atol, int64_t dim, const uint64_t* reps,
+                              const int64_t* stab, int nthreads, int64_t* first_bad) {
+    const oc_group G = group_view(n_site, n_group, perm, phases, fermionic);
+    const cplx* chi = (const cplx*)chi_;
+    int64_t bad = 0, first = -1;
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static, 4096) reduction(+ : bad)
+    for (int64_t r = 0; r < dim; ++r) {
+        const mask_t m = reps[r];
+        int ok = __builtin_popcountll(m) == n_filled && (n_site == 64 || (m >> n_site) == 0);
+        if (ok && r > 0 && !(reps[r - 1] < m)) ok = 0;
+        int64_t so = 0;
+        for (int g = 0; g < n_group && ok; ++g) {
+            cplx a;
+            const mask_t s = apply_op(m, &G, g, &a);
+            if (s < m) ok = 0;
+            else if (s == m) {
+                ++so;
+                if (!(cabs(chi[g] - a) <= atol)) ok = 0;
+            }
+        }
+        if (ok && so != stab[r]) ok = 0;
+        if (!ok) {
+            ++bad;
+#pragma omp critical
+            if (first < 0 || r < first) first = r;
+        }
+    }
+    *first_bad = first;
+    return bad;
+}
+
+static int64_t find_rep(const mask_t* reps, int64_t dim, mask_t m) {
+    int64_t lo = 0, hi = dim - 1;
+    while (lo <= hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (reps[mid] == m) return mid;
+        if (reps[mid] < m) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+static int is_canonical(mask_t m, const oc_group* G) {
+    for (int g = 0; g < G->n_group; ++g) {
+        cplx a;
+        if (apply_op(m, G, g, &a) < m) return 0;
+    }
+    return 1;
+}
+
+/* (H x)[r] for the listed rows r (0-based) of a sector given by its basis (checked with
+ * oc_check_sector_basis), without rep tables.  H[r, c] is the entry
+ * build_matrixfree_projection_table (:680-728) writes for column c: term k = (i_from,
+ * i_to, t) hops the column's representative m_c to nm = m_c xor hop_k when i_from is
+ * occupied and i_to empty (:709-710), nm projects onto row r when its canonical
+ * representative (get_canonical_representative, :174-187, via g with alpha = the
+ * phase of g on nm) is rep_r, and the entry is ((t * hop_phase) * alpha conj(chi(g))) *
+ * sqrt(stab_r / stab_c) (:718).  So the columns reaching row r are found from the row:
+ * for each distinct image s of rep_r and each term with i_to occupied and i_from empty
+ * in s, m_c = s xor hop_k must itself be a representative in the basis.  The diagonal
+ * is the sum of V over density terms with both sites occupied (:700-706).  No use is
+ * made of Hermiticity. */
+void oc_sample_rows(int n_site, int n_group, const int32_t* perm, const double* phases, int fermionic,
+                    const double* chi_, int64_t dim, const uint64_t* reps, const int64_t* stab, int n_bil,
+                    const int32_t* ifrom, const int32_t* ito, const double* t_, int n_den, const int32_t* di,
+                    const int32_t* dj, const double* V_, const double* x_, int64_t n_rows, const int64_t* rows,
+                    double* y_, int nthreads) {
+    const oc_group G = group_view(n_site, n_group, perm, phases, fermionic);
+    const cplx* chi = (const cplx*)chi_;
+    const cplx* t = (const cplx*)t_;
+    const cplx* V = (const cplx*)V_;
+    const cplx* x = (const cplx*)x_;
+    cplx* y = (cplx*)y_;
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16)
+    for (int64_t q = 0; q < n_rows; ++q) {
+        const int64_t r = rows[q];
+        const mask_t mr = reps[r];
+        cplx acc = 0.0;
+        cplx hd = 0.0;
+        for (int p = 0; p < n_den; ++p)
+            if (((mr >> (di[p] - 1)) & 1) && ((mr >> (dj[p] - 1)) & 1)) hd = hd + V[p];
+        acc = hd * x[r];
+        /* distinct images of rep_r */
+        mask_t img[64];
+        int n_img = 0;
+        for (int g = 0; g < n_group; ++g) {
+            cplx a;
+            const mask_t s = apply_op(mr, &G, g, &a);
+            int seen = 0;
+            for (int u = 0; u < n_img && !seen; ++u) seen = img[u] == s;
+            if (!seen) img[n_img++] = s;
+        }
+        for (int u = 0; u < n_img; ++u) {
+            const mask_t s = img[u];
+            /* get_canonical_representative of s: first minimal element (strict <) */
+            mask_t repr = s;
+            int gbest = 0;
+            cplx abest = 1.0;
+            for (int g = 0; g < n_group; ++g) {
+                cplx a;
+                const mask_t w = apply_op(s, &G, g, &a);
+                if (w < repr) { repr = w; gbest = g; abest = a; }
+            }
+            const cplx coeff = abest * conj(chi[gbest]);
+            for (int k = 0; k < n_bil; ++k) {
+                const int f = ifrom[k] - 1, to = ito[k] - 1;
+                if (f == to) continue;
+                if (!((s >> to) & 1) || ((s >> f) & 1)) continue;
+                const mask_t mc = s ^ (((mask_t)1 << f) | ((mask_t)1 << to));
+                if (!is_canonical(mc, &G)) continue;
+                const int64_t c = find_rep(reps, dim, mc);
+                if (c < 0) continue;
+                const double hp = hop_phase(fermionic, mc, ifrom[k], ito[k]);
+                const double nrm = sqrt((double)stab[r] / (double)stab[c]);
+                const cplx h = ((t[k] * hp) * coeff) * nrm;
+                acc += h * x[c];
+            }
+        }
+        y[q] = acc;
+    }
+}

@@ -1,16 +1,25 @@
 """The benchmark workloads at their full sizes, through the C ABI.
 
-Where the oracle cannot follow (sectors of 4M and 15.7M states), parity rests on
-size-independent properties: sector dimensions against the character formula (closed
-form C1) and their sum against C(N, k); the matvec's Hermiticity and linearity; FP32
-against FP64; FP32+refine against FP64 eigenvalues.  Config[0] (XXZ N=16, dim 810) is
-small enough for the oracle's dense spectrum.
+The oracle's catalog cannot be built at these sizes (C(48, 8) = 377M ranks), so parity
+rests on the table-free oracle and on size-independent properties:
+  * the GPU's sector basis checked entry by entry against its definition (canonical
+    representative, stabilizer order, stabilizer phases = characters, ascending) with
+    the character-formula length, which together fix the list;
+  * (H x)[r] on 10,000 sampled rows from the oracle's row restatement of the projection
+    table (oc_sample_rows: no rep tables, no Hermiticity) against the GPU matvec, FP64
+    and FP32, for every launch shape on the checkerboard 4x6 workload;
+  * eigenpairs: the oracle's rows of H v equal lambda v on the sampled rows, plus
+    residuals, orthonormality, other seeds and FP32+refine against FP64;
+  * sector dimensions against the character formula (closed form C1) and their sum
+    against C(N, k); Hermiticity and linearity of the matvec; FP32 against FP64.
+Config[0] (XXZ N=16, dim 810) is small enough for the oracle's dense spectrum.
 """
 import math
 
 import numpy as np
 import pytest
 
+from realspace_exactdiagonalization_amd import _lib
 from realspace_exactdiagonalization_amd import lattice as L
 from tests.test_oracle import character_dim
 
@@ -28,6 +37,159 @@ def ctx(ed):
     c = ed.Context(0)
     yield c
     c.close()
+
+
+N_ROWS = 10000
+
+
+class FullSector:
+    """Catalog, sector and terms of one workload at its full size, with the basis dump
+    checked against its definition once."""
+
+    def __init__(self, ed, ctx, m, theta):
+        self.m = L.twisted(m, theta)
+        self.G = L.model_group(self.m, theta)
+        self.ir = L.translation_irreps(self.G, self.m.lattice)[0]
+        self.cat = catalog(ed, ctx, self.m, self.G)
+        self.b = ed.build_symmetry_sector_basis(self.cat, self.ir)
+        self.H = ed.MatrixFreeHamiltonian(self.b, self.m.bilinear_terms, self.m.density_terms)
+        self.reps = self.b.representative_mask_list
+        self.stab = self.b.stabilizer_order_list
+        self.checked = None
+
+    def check_basis(self, oracle_mod):
+        if self.checked is None:
+            bad = oracle_mod.check_sector_basis(self.G, self.m.n_filled, self.m.statistics == L.FERMIONIC,
+                                                self.ir.values, self.reps, self.stab)
+            self.checked = (bad, len(self.reps) == round(character_dim(self.m, self.G, self.ir.values)))
+        return self.checked
+
+    def rows(self, seed=0):
+        rng = np.random.default_rng(seed)
+        r = rng.choice(self.b.dim, size=min(N_ROWS, self.b.dim), replace=False)
+        return np.unique(np.concatenate([r, [0, self.b.dim - 1]]))
+
+    def oracle_rows(self, oracle_mod, x, rows):
+        return oracle_mod.sample_rows(self.m, self.G, self.ir.values, self.reps, self.stab, x, rows)
+
+    def close(self):
+        self.b.close()
+        self.cat.close()
+
+
+FULL = {"checker4x6": lambda: L.workload("checker4x6"), "hubbard4x4": lambda: L.workload("hubbard4x4"),
+        "xxz32": lambda: L.heisenberg_chain(32)}
+
+
+@pytest.fixture(scope="module")
+def full(ed, ctx):
+    cache = {}
+
+    def get(name, theta):
+        key = (name, tuple(theta))
+        if key not in cache:
+            for v in cache.values():   # one workload on the device at a time
+                v.close()
+            cache.clear()
+            print(f"[full] building {name} theta={theta}", flush=True)
+            cache[key] = FullSector(ed, ctx, FULL[name](), theta)
+        return cache[key]
+
+    yield get
+    for v in cache.values():
+        v.close()
+
+
+FULL_CASES = [("checker4x6", (0.0, 0.0)), ("checker4x6", (0.3, 0.7)), ("hubbard4x4", (0.0, 0.0)),
+              ("xxz32", (0.0, 0.0))]
+
+
+@pytest.mark.parametrize("name,theta", FULL_CASES)
+def test_full_size_sector_basis_matches_its_definition(oracle_mod, full, name, theta):
+    fs = full(name, theta)
+    print(f"[full-basis] {name} theta={theta} dim={fs.b.dim}", flush=True)
+    (bad, first), length_ok = fs.check_basis(oracle_mod)
+    assert (bad, first) == (0, -1)
+    assert length_ok
+
+
+@pytest.mark.parametrize("name,theta", FULL_CASES)
+def test_full_size_sampled_rows_match_the_oracle(ed, oracle_mod, full, name, theta):
+    """GPU H x on the autotuned launch shape against the oracle's rows, FP64 (1e-12) and
+    FP32 (5e-6), relative to the largest sampled |(H x)[r]|."""
+    fs = full(name, theta)
+    assert fs.check_basis(oracle_mod) == ((0, -1), True)
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(fs.b.dim) + 1j * rng.standard_normal(fs.b.dim)
+    rows = fs.rows(1)
+    ref = fs.oracle_rows(oracle_mod, x, rows)
+    scale = np.abs(ref).max()
+    for prec, tol in ((ed.PREC_FP64, 1e-12), (ed.PREC_FP32, 5e-6)):
+        fs.H.precision = prec
+        y = fs.H(x)
+        err = np.abs(y[rows] - ref).max() / scale
+        print(f"[full-rows] {name} theta={theta} prec={prec} shape={fs.b.shape(prec)['shape']} "
+              f"rel_err={err:.2e}", flush=True)
+        assert err <= tol
+    fs.H.precision = ed.PREC_FP64
+
+
+def test_full_size_every_launch_shape_matches_the_oracle(ed, oracle_mod, full):
+    """Every launch shape on the checkerboard 4x6 workload (48 sites, 64-bit masks,
+    fermions) against the oracle's rows.  The flat hop loop needs C(47, 9) < 2^32 and is
+    refused there with SED_EUNSUPPORTED."""
+    fs = full("checker4x6", (0.0, 0.0))
+    assert fs.check_basis(oracle_mod) == ((0, -1), True)
+    rng = np.random.default_rng(23)
+    x = rng.standard_normal(fs.b.dim) + 1j * rng.standard_normal(fs.b.dim)
+    rows = fs.rows(2)
+    ref = fs.oracle_rows(oracle_mod, x, rows)
+    scale = np.abs(ref).max()
+    n_shapes = len(fs.b.shape(ed.PREC_FP64)["tune_ms"])
+    ran = 0
+    for prec, tol in ((ed.PREC_FP64, 1e-12), (ed.PREC_FP32, 5e-6)):
+        fs.H.precision = prec
+        for sh in range(n_shapes):
+            fs.b.set_shape(prec, sh, 64)
+            try:
+                y = fs.H(x)
+            except _lib.SedError as e:
+                assert e.code == _lib.SED_EUNSUPPORTED, (sh, e)
+                print(f"[full-shape] prec={prec} shape={sh} refused: {e}", flush=True)
+                continue
+            err = np.abs(y[rows] - ref).max() / scale
+            print(f"[full-shape] prec={prec} shape={sh} rel_err={err:.2e}", flush=True)
+            assert err <= tol, (prec, sh)
+            ran += 1
+        fs.b.set_shape(prec, -1)
+    fs.H.precision = ed.PREC_FP64
+    assert ran >= 2 * 9
+
+
+@pytest.mark.parametrize("name", ["checker4x6", "hubbard4x4"])
+def test_full_size_eigenpairs_satisfy_the_oracle_rows(ed, oracle_mod, full, name):
+    """The lowest three eigenpairs of the k = (0,0) sector, FP64 and FP32+refine: on the
+    sampled rows the oracle's (H v)[r] equals lambda v[r] (2e-10, the solver's residual
+    tolerance being 1e-10), the vectors are orthonormal, FP32+refine and a second seed
+    give the FP64 eigenvalues to 1e-10 relative."""
+    fs = full(name, (0.0, 0.0))
+    assert fs.check_basis(oracle_mod) == ((0, -1), True)
+    rows = fs.rows(3)
+    out = {}
+    for prec, seed in ((ed.PREC_FP64, 1234), (ed.PREC_FP32_REFINE, 1234), (ed.PREC_FP64, 99)):
+        st = {}
+        vals, vecs = ed.diagonalize_block_matrixfree(fs.H, fs.b.dim, nev=3, precision=prec, seed=seed, stats=st)
+        print(f"[full-eig] {name} prec={prec} seed={seed} vals={vals.tolist()} "
+              f"steps={st['iters_fp32']}+{st['iters_fp64']} resid={st['residual_max']:.2e}", flush=True)
+        gram = vecs.conj().T @ vecs
+        assert np.abs(gram - np.eye(3)).max() <= 1e-10
+        for j in range(3):
+            hv = fs.oracle_rows(oracle_mod, vecs[:, j], rows)
+            assert np.abs(hv - vals[j] * vecs[rows, j]).max() <= 2e-10, (prec, j)
+        out[(prec, seed)] = vals
+    ref = out[(ed.PREC_FP64, 1234)]
+    for v in out.values():
+        assert np.abs(v - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
 def catalog(ed, ctx, m, G):

@@ -506,3 +506,91 @@ def test_oracle_structure_factor_demo(oracle_mod, kind, tpp):
     C, d = sec.density_correlation(v[:, 0])
     k, S = k_map(m.lattice, C - np.outer(d, d))
     sq_checks(k, S)
+
+
+# ---------------------------------------------------------------- table-free checkers
+SAMPLE_CASES = {
+    "checker3x2_f_flux": (lambda: L.checkerboard(3, 2, n_filled=4, V1=1.0, V2=0.5), (0.23, 0.17)),
+    "checker3x2_b_flux": (lambda: L.checkerboard(3, 2, n_filled=4, statistics=L.BOSONIC), (0.23, 0.17)),
+    "hub2x3": (lambda: L.hubbard_square(2, 3, U=4.0, n_filled=6), (0.0, 0.0)),
+    "hald3x2_flux": (lambda: L.haldane_honeycomb(3, 2, n_filled=3, V1=0.3), (0.4, 0.0)),
+    "heis14": (lambda: L.heisenberg_chain(14), (0.0, 0.0)),
+    # 48 sites: masks above bit 31, fermion signs from the upper word
+    "checker4x6_k3_flux": (lambda: L.checkerboard(4, 6, n_filled=3), (0.3, 0.7)),
+}
+
+
+@pytest.mark.parametrize("name", list(SAMPLE_CASES))
+def test_sampled_rows_equal_the_projection_table(oracle_mod, name):
+    """oc_sample_rows (rows found from the row side, no rep tables, no Hermiticity) equals
+    _apply_table! over the reference-layout projection table on every row of every sector."""
+    f, th = SAMPLE_CASES[name]
+    m = L.twisted(f(), th)
+    G = L.model_group(m, th)
+    cat = oracle_mod.Catalog(m.n_site, m.n_filled, G, m.statistics == L.FERMIONIC)
+    rng = np.random.default_rng(17)
+    checked = 0
+    for ir in L.translation_irreps(G, m.lattice):
+        sec = oracle_mod.Sector(cat, ir.values)
+        if sec.dim == 0:
+            continue
+        sec.set_terms(*L.term_arrays(m))
+        reps, stab = sec.basis()
+        x = rng.standard_normal(sec.dim) + 1j * rng.standard_normal(sec.dim)
+        ref = sec.apply_table(x, 1)
+        rows = np.arange(sec.dim)
+        y = oracle_mod.sample_rows(m, G, ir.values, reps, stab, x, rows)
+        assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max(), ir.label
+        checked += 1
+    assert checked >= 4
+
+
+@pytest.mark.parametrize("name", ["checker3x2_f_flux", "checker3x2_b_flux", "hub2x3", "checker4x6_k3_flux"])
+def test_sector_basis_checker(oracle_mod, name):
+    """check_sector_basis accepts every sector basis the oracle catalog gives and finds each
+    planted defect: a non-canonical orbit member in place of its representative, a wrong
+    stabilizer order, two entries swapped, a representative of an incompatible orbit, a
+    mask with one particle too many."""
+    f, th = SAMPLE_CASES[name]
+    m = L.twisted(f(), th)
+    G = L.model_group(m, th)
+    fermi = m.statistics == L.FERMIONIC
+    cat = oracle_mod.Catalog(m.n_site, m.n_filled, G, fermi)
+    all_reps, all_stab, _, _ = cat.orbits()
+    irs = L.translation_irreps(G, m.lattice)
+    bases = []
+    for ir in irs:
+        sec = oracle_mod.Sector(cat, ir.values)
+        reps, stab = sec.basis()
+        assert oracle_mod.check_sector_basis(G, m.n_filled, fermi, ir.values, reps, stab) == (0, -1)
+        assert len(reps) == round(character_dim(m, G, ir.values))
+        bases.append((ir, reps, stab))
+    ir, reps, stab = max(bases, key=lambda b: len(b[1]))
+    j = len(reps) // 2
+    # a non-canonical member of the same orbit
+    for g in range(2, G.order + 1):
+        img, _ = cat.apply_operation(int(reps[j]), g)
+        if img != int(reps[j]):
+            break
+    bad = reps.copy()
+    bad[j] = img
+    assert oracle_mod.check_sector_basis(G, m.n_filled, fermi, ir.values, bad, stab)[0] >= 1
+    st = stab.copy()
+    st[j] += 1
+    assert oracle_mod.check_sector_basis(G, m.n_filled, fermi, ir.values, reps, st) == (1, j)
+    sw = reps.copy()
+    sw[[j, j + 1]] = sw[[j + 1, j]]
+    assert oracle_mod.check_sector_basis(G, m.n_filled, fermi, ir.values, sw, stab)[0] >= 1
+    # a representative that only another irrep keeps (a stabilizer phase differs)
+    mine = set(int(r) for r in reps)
+    others = [(int(r), int(s)) for r, s in zip(all_reps, all_stab) if int(r) not in mine]
+    if others:
+        r0, s0 = others[0]
+        ins = np.searchsorted(reps, np.uint64(r0))
+        ext = np.insert(reps, ins, np.uint64(r0))
+        exs = np.insert(stab, ins, s0)
+        assert oracle_mod.check_sector_basis(G, m.n_filled, fermi, ir.values, ext, exs) == (1, int(ins))
+    extra = reps.copy()
+    free_bit = next(b for b in range(m.n_site) if not (int(reps[-1]) >> b) & 1)
+    extra[-1] = np.uint64(int(reps[-1]) | (1 << free_bit))
+    assert oracle_mod.check_sector_basis(G, m.n_filled, fermi, ir.values, extra, stab)[0] >= 1
