@@ -167,39 +167,6 @@ def flux_scan(E, L, ctx, n_twists=16, nev=3, lanes=8):
     return out
 
 
-# Serving-stream request families: XXZ chains at half filling, 2 x L Hubbard ladders at
-# quarter filling (U = 4), checkerboard lattices at 1/3 filling of the lower band.
-STREAM_SIZES = {"xxz": [20, 22, 24, 26, 28, 30, 32], "ladder": [6, 7, 8, 9],
-                "checker": [(3, 4), (3, 5), (3, 6), (4, 4), (4, 5), (4, 6)]}
-
-
-def stream_model(L, family, size):
-    if family == "xxz":
-        return L.heisenberg_chain(size)
-    if family == "ladder":
-        return L.hubbard_square(size, 2, U=4.0, t=1.0, n_filled=size)
-    return L.checkerboard(size[0], size[1])
-
-
-def stream_requests(n, seed):
-    """Seeded request list: family and size uniform, momentum draw u in [0, 1) (irrep index
-    floor(u * |G|)), twist uniform in [0, 1)^2 (the chain uses only its x component)."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    fams = ["xxz", "ladder", "checker"]
-    out = []
-    for _ in range(n):
-        f = fams[int(rng.integers(3))]
-        sizes = STREAM_SIZES[f]
-        size = sizes[int(rng.integers(len(sizes)))]
-        u = float(rng.random())
-        th = (float(rng.random()), float(rng.random()))
-        if f == "xxz":
-            th = (th[0], 0.0)
-        out.append({"family": f, "size": size, "u": u, "theta": th})
-    return out
-
-
 def poisson_arrivals(n, rate, seed):
     """Arrival times (s) of a Poisson stream of n requests at `rate` per second, first at 0."""
     import numpy as np
@@ -220,16 +187,14 @@ def serving_stream(E, L, ctx, n_req=32, seed=2024, nev=3, lanes=8, load=1.2):
     Reports solves/s and p50/p99 latency."""
     import threading
     import numpy as np
-    reqs = stream_requests(n_req, seed)
+    reqs = L.stream_requests(n_req, seed)
     t0 = time.time()
     keep, sectors, info = [], [], []
     for r in reqs:
-        m = L.twisted(stream_model(L, r["family"], r["size"]), r["theta"])
-        G = L.model_group(m, r["theta"])
+        m, G, ir = L.stream_sector(r)
         cat = E.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=m.n_filled, symmetry_group=G,
                                              particle_statistics=m.statistics, ctx=ctx)
-        irs = L.translation_irreps(G, m.lattice)
-        b = E.build_symmetry_sector_basis(cat, irs[min(len(irs) - 1, int(r["u"] * len(irs)))])
+        b = E.build_symmetry_sector_basis(cat, ir)
         b.set_terms(m.bilinear_terms, m.density_terms)
         keep.append(cat)
         if b.dim == 0:
@@ -319,7 +284,7 @@ def serving_stream(E, L, ctx, n_req=32, seed=2024, nev=3, lanes=8, load=1.2):
         b.close()
     q = lambda a, p: round(1e3 * float(np.percentile(a, p)), 2)
     return {
-        "requests": n, "families": {f: int(sum(1 for x in info if x[0] == f)) for f in STREAM_SIZES},
+        "requests": n, "families": {f: int(sum(1 for x in info if x[0] == f)) for f in L.STREAM_SIZES},
         "dims": [int(dims.min()), int(np.median(dims)), int(dims.max())], "nev": nev, "seed": seed,
         "build_s": round(t_build, 2), "tune_s": round(t_tune, 2),
         "plain_fp64": {"solves_per_s": round(plain_rate, 3), "solve_s_total": round(float(t_solve.sum()), 3),

@@ -58,7 +58,8 @@ typedef struct sed_solve_stats {
     double residual_max;      /* max residual norm of the returned pairs */
     double seconds;           /* wall time of the solve */
     int32_t full_orth_steps;  /* steps orthogonalised against the whole Krylov basis */
-    int32_t reserved;
+    int32_t repeated;         /* 1 if the FP64 phase failed its true-residual check and was
+                                 repeated with whole-basis passes (sed_solve, nev <= 5) */
 } sed_solve_stats;
 
 /* Thread-local message of the last failing call on this thread. */
@@ -208,8 +209,13 @@ int sed_sector_destroy(sed_sector* sec);
  * flux scan).  Requests are admitted while their HBM footprint fits the budget
  * (hbm_budget_bytes <= 0: 80% of free memory) and one of max_slots solver lanes is
  * free; each lane has its own HIP stream, so admitted requests overlap on the
- * device, and a finished request frees its lane for the queue at once.  Two requests
- * on the same sector object never run at the same time. */
+ * device, and a finished request frees its lane for the queue at once.  The budget
+ * covers all device memory the server allocates: the packed engine's arena (a quarter
+ * of the budget, at most 16 GiB, taken at creation), the workspaces lanes keep between
+ * requests (released when a queued request needs the room) and the running requests'
+ * footprints; catalogs and sectors belong to the caller and are not counted.  A request
+ * whose footprint exceeds what the arena leaves of the budget fails with SED_ENOMEM.
+ * Two requests on the same sector object of one server never run at the same time. */
 int sed_server_create(sed_context* ctx, int64_t hbm_budget_bytes, int max_slots, sed_server** out);
 /* Queue one solve of `sec` (which must stay alive until its result is taken). */
 int sed_server_submit(sed_server* srv, sed_sector* sec, int nev, double tol, uint64_t seed, int maxiter,
@@ -227,6 +233,11 @@ int sed_server_run(sed_server* srv, int max_steps, int64_t* n_done);
  * long-running service recreates its server from time to time. */
 int sed_server_result(sed_server* srv, int64_t request_id, int* status, double* evals, sed_solve_stats* stats,
                       double* latency_s);
+/* Memory accounting of a server (no reference counterpart): its HBM budget, the bytes
+ * taken at creation (packed engine), the bytes charged now (creation + lane workspaces +
+ * running footprints) and the largest such charge so far; any pointer may be NULL. */
+int sed_server_memory(sed_server* srv, int64_t* budget_bytes, int64_t* fixed_bytes, int64_t* in_use_bytes,
+                      int64_t* peak_bytes);
 int sed_server_destroy(sed_server* srv);
 
 #ifdef __cplusplus

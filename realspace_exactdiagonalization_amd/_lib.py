@@ -46,7 +46,7 @@ class SedError(RuntimeError):
 class SolveStats(C.Structure):
     _fields_ = [("converged", C.c_int32), ("iters_fp32", C.c_int32), ("iters_fp64", C.c_int32),
                 ("restarts", C.c_int32), ("matvecs", C.c_int64), ("residual_max", C.c_double),
-                ("seconds", C.c_double), ("full_orth_steps", C.c_int32), ("reserved", C.c_int32)]
+                ("seconds", C.c_double), ("full_orth_steps", C.c_int32), ("repeated", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -96,6 +96,8 @@ SIGNATURES = {
     "sed_server_run": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64)]),
     "sed_server_result": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_int), _f64p, C.POINTER(SolveStats),
                                     C.POINTER(C.c_double)]),
+    "sed_server_memory": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int64)]),
     "sed_server_destroy": (C.c_int, [_vp]),
 }
 

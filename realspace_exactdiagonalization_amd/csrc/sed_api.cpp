@@ -427,6 +427,7 @@ int sed_sector_set_terms(sed_sector* sec, int n_bil, const int32_t* i_from, cons
     const sed_catalog* cat = sec->cat;
     const int N = cat->n_site;
     SED_HIP(hipSetDevice(cat->ctx->device));
+    const int old_terms = sec->has_terms ? sec->n_terms : -1, old_dens = sec->has_terms ? sec->n_dens : -1;
     // merge hops with equal (from, to): their matrix contributions add
     std::map<std::pair<int, int>, cd> hops;
     for (int k = 0; k < n_bil; ++k) {
@@ -498,6 +499,13 @@ int sed_sector_set_terms(sed_sector* sec, int n_bil, const int32_t* i_from, cons
         return fail(SED_EUNSUPPORTED, "term lists need " + std::to_string(need) + " bytes of LDS per workgroup");
     sec->has_terms = true;
     sec->hops_gated = sec->hops_in_sector = -1;
+    if (sec->n_terms != old_terms || sec->n_dens != old_dens) {
+        // a launch shape chosen (and LDS-checked) for other term lists: choose again
+        for (int p = 0; p < 2; ++p) {
+            sec->mv_shape[p] = -1;
+            sec->mv_chunk[p] = 1;
+        }
+    }
     sec->csc_ready = false;   // a kept matrix belongs to the previous terms
     std::vector<int64_t>().swap(sec->csc_colptr);
     std::vector<int64_t>().swap(sec->csc_rowval);

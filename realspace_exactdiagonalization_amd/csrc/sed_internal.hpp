@@ -252,6 +252,7 @@ class PackEngine;
 PackEngine* pack_create(sed_context* ctx, int64_t arena_bytes, int max_active, void (*on_done)(void*, PackJob*),
                         void* user);
 void pack_submit(PackEngine* e, PackJob* job);
+int64_t pack_device_bytes(const PackEngine* e);
 void pack_destroy(PackEngine* e);
 bool pack_eligible(const sed_sector* sec, int nev);
 int64_t pack_footprint(const sed_sector* sec, int nev, int precision);

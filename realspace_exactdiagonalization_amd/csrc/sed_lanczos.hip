@@ -364,22 +364,32 @@ static inline unsigned grid_for(int64_t n) {
 // ------------------------------------------------------------------------
 // host driver
 // ------------------------------------------------------------------------
+// The pinned host block of a workspace is allocated once, at the largest size any solve
+// uses (the alpha/beta/flag block of the largest basis, or the fixed reduction grid), and
+// kept until release(): it is never re-allocated between or during solves, so no copy in
+// flight can land in freed pinned memory.
+constexpr size_t kHostDoubles = std::max<size_t>(4 * (kMaxBasis + 2) + 2 * kMaxBasis + 1 + 8, kUpdGrid);
+
 int Workspace::ensure_host(size_t n_doubles) {
-    if (hcoef_n >= n_doubles) return SED_OK;
-    if (hcoef) (void)hipHostFree(hcoef);
-    hcoef = nullptr;
-    hipError_t e = hipHostMalloc((void**)&hcoef, n_doubles * sizeof(double), hipHostMallocDefault);
+    if (n_doubles > kHostDoubles) return fail(SED_EINVAL, "pinned host block too small");
+    if (hcoef) return SED_OK;
+    hipError_t e = hipHostMalloc((void**)&hcoef, kHostDoubles * sizeof(double), hipHostMallocDefault);
     if (e != hipSuccess) {
         hcoef = nullptr;
         hcoef_n = 0;
         return fail(SED_ENOMEM, std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
     }
-    hcoef_n = n_doubles;
+    hcoef_n = kHostDoubles;
     return SED_OK;
 }
 
+// A captured step is destroyed only after the stream has drained, so no replay of it is
+// still queued or running when its executable graph goes away.
 void Workspace::drop_graph() {
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph_exec) {
+        if (stream) (void)hipStreamSynchronize(stream);
+        (void)hipGraphExecDestroy(graph_exec);
+    }
     graph_exec = nullptr;
     graph_key = GraphKey();
 }
@@ -890,6 +900,7 @@ static int verify_fp64(sed_sector* sec, int m, int nev, double tol, uint64_t see
         SED_TRY(F.start_from(v0));
     }
     // the first run's work is counted here, the repeated run's by the caller
+    st.repeated = 1;
     st.iters_fp64 += o.iters;
     st.matvecs += o.matvecs;
     st.restarts += o.restarts;

@@ -379,6 +379,7 @@ class Arena {
         }
     }
     uint8_t* base() { return buf_.p; }
+    int64_t capacity() const { return (int64_t)buf_.bytes(); }
     void destroy() { buf_.release(); }
 
    private:
@@ -917,6 +918,14 @@ class PackEngine {
     std::thread th_;
     std::mutex mu_;
     std::condition_variable cv_;
+   public:
+    // device memory the engine holds from start() to its destruction
+    int64_t device_bytes() const {
+        return (int64_t)(arena_.capacity() + slots_.bytes() + pros_.bytes() + items_.bytes() + moves_.bytes() +
+                         coef_.bytes());
+    }
+
+   private:
     std::deque<PackJob*> queue_;
     bool stop_ = false;
     Arena arena_;
@@ -943,6 +952,7 @@ PackEngine* pack_create(sed_context* ctx, int64_t arena_bytes, int max_active, v
     return e;
 }
 void pack_submit(PackEngine* e, PackJob* job) { e->submit(job); }
+int64_t pack_device_bytes(const PackEngine* e) { return e ? e->device_bytes() : 0; }
 void pack_destroy(PackEngine* e) {
     if (!e) return;
     e->stop();

@@ -7,6 +7,15 @@
 // request is admitted as soon as a lane is free and its device footprint fits the HBM
 // budget; lanes pick up new work the moment they finish, so requests submitted while
 // others run are admitted without waiting for the batch to drain.
+//
+// Memory accounting: the budget covers everything the server allocates.  The packed
+// engine's arena and its fixed buffers are charged once, at creation.  Each lane is
+// charged what its workspace holds while idle, and while running what it held before
+// plus the request's footprint (a workspace buffer only grows to the size the request
+// needs, so that sum bounds it).  When a request does not fit, idle lanes' kept
+// workspaces are released first; a request larger than the whole lane share of the
+// budget fails at once with SED_ENOMEM.  Catalogs and sectors belong to the caller and
+// are not counted.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -22,6 +31,7 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 constexpr size_t kLaneKeepBytes = (size_t)1 << 30;   // idle lane workspace kept for reuse
+constexpr int kPackMaxActive = 512;                   // sectors in flight in the packed engine
 
 struct Request {
     int64_t id = 0;
@@ -40,7 +50,8 @@ struct Request {
 struct Lane {
     std::thread th;
     Workspace ws;
-    int64_t job = -1;  // request id assigned, -1 idle
+    int64_t job = -1;   // request id assigned, -1 idle
+    int64_t charge = 0; // bytes charged to the budget for this lane
 };
 
 }  // namespace
@@ -48,7 +59,8 @@ struct Lane {
 
 struct sed_server {
     sed_context* ctx = nullptr;
-    int64_t budget = 0;
+    int64_t budget = 0;       // everything the server allocates
+    int64_t fixed = 0;        // packed engine arena + buffers, charged at creation
     sed::PackEngine* pack = nullptr;  // small sectors: packed persistent launches
     std::unordered_map<int64_t, std::unique_ptr<sed::PackJob>> pack_jobs;
     std::mutex mu;
@@ -57,7 +69,8 @@ struct sed_server {
     std::unordered_map<int64_t, sed::Request> reqs;
     std::vector<sed::Lane*> lanes;
     int64_t next_id = 1;
-    int64_t in_use = 0;
+    int64_t in_use = 0;       // fixed + the lanes' charges
+    int64_t peak = 0;         // largest in_use so far
     int64_t n_done = 0;
     int64_t n_submitted = 0;
     bool stop = false;
@@ -67,6 +80,39 @@ struct sed_server {
 namespace sed {
 namespace {
 
+void set_in_use(sed_server* srv) {
+    int64_t u = srv->fixed;
+    for (const Lane* l : srv->lanes) u += l->charge;
+    srv->in_use = u;
+    srv->peak = std::max(srv->peak, u);
+}
+
+void release_lane(sed_server* srv, Lane* l) {
+    (void)hipSetDevice(srv->ctx->device);
+    l->ws.release();
+    l->charge = 0;
+    set_in_use(srv);
+}
+
+// Charge request r to the idle lane: what the lane keeps plus r's footprint.  When that
+// does not fit, the lane's own kept buffers are released, then other idle lanes' (idle
+// lanes have no work on their streams: a solve ends with a stream synchronisation, so
+// their buffers can be freed from this thread).  False, with nothing released, if r
+// cannot fit while the running requests hold their memory.
+bool fit_locked(sed_server* srv, Lane* lane, const Request& r) {
+    if (srv->in_use + r.footprint <= srv->budget) return true;
+    int64_t idle = 0;
+    for (const Lane* l : srv->lanes)
+        if (l->job < 0) idle += l->charge;
+    if (srv->in_use - idle + r.footprint > srv->budget) return false;
+    if (lane->charge > 0) release_lane(srv, lane);
+    for (Lane* l : srv->lanes) {
+        if (srv->in_use + r.footprint <= srv->budget) break;
+        if (l->job < 0 && l->charge > 0) release_lane(srv, l);
+    }
+    return srv->in_use + r.footprint <= srv->budget;
+}
+
 // Assign queued requests to idle lanes while they fit (caller holds the lock).
 void admit_locked(sed_server* srv) {
     for (size_t li = 0; li < srv->lanes.size(); ++li) {
@@ -74,12 +120,13 @@ void admit_locked(sed_server* srv) {
         if (lane->job >= 0) continue;
         for (auto it = srv->queue.begin(); it != srv->queue.end(); ++it) {
             Request& r = srv->reqs[*it];
-            if (r.sec->busy) continue;                              // one solve per sector at a time
-            if (srv->in_use + r.footprint > srv->budget) continue;  // wait for memory
+            if (r.sec->busy) continue;   // one solve per sector at a time
+            if (!fit_locked(srv, lane, r)) continue;
             r.status = 2;
             r.t_start = clk::now();
             r.sec->busy = 1;
-            srv->in_use += r.footprint;
+            lane->charge += r.footprint;
+            set_in_use(srv);
             lane->job = r.id;
             srv->queue.erase(it);
             break;
@@ -104,7 +151,7 @@ void lane_main(sed_server* srv, Lane* lane) {
         sed_solve_stats st{};
         const int rc = solve_sector(sec, nev, tol, seed, maxiter, prec, ev, nullptr, &st, lane->ws);
         const std::string err = rc == SED_OK ? std::string() : std::string(sed_last_error());
-        // the admission budget counts running requests only: give large idle buffers back
+        // keep up to kLaneKeepBytes of workspace for the next request (charged while idle)
         if (lane->ws.device_bytes() > kLaneKeepBytes) lane->ws.release();
         lk.lock();
         Request& r2 = srv->reqs[lane->job];
@@ -114,7 +161,8 @@ void lane_main(sed_server* srv, Lane* lane) {
         r2.error = err;
         r2.t_end = clk::now();
         r2.sec->busy = 0;
-        srv->in_use -= r2.footprint;
+        lane->charge = (int64_t)lane->ws.device_bytes();
+        set_in_use(srv);
         ++srv->n_done;
         lane->job = -1;
         admit_locked(srv);
@@ -132,7 +180,9 @@ void pack_done(void* user, PackJob* job) {
     r.status = job->status;
     r.error = job->error;
     r.t_end = clk::now();
+    r.sec->busy = 0;
     ++srv->n_done;
+    admit_locked(srv);   // a queued request on the same sector may start now
     srv->cv_done.notify_all();
 }
 
@@ -174,13 +224,15 @@ int sed_server_create(sed_context* ctx, int64_t hbm_budget_bytes, int max_slots,
     for (Lane* lane : srv->lanes) lane->th = std::thread(lane_main, srv, lane);
     if (!pack_disabled()) {
         const int64_t arena = std::min<int64_t>(srv->budget / 4, (int64_t)16 << 30);
-        srv->pack = pack_create(ctx, arena, 512, pack_done, srv);
+        srv->pack = pack_create(ctx, arena, kPackMaxActive, pack_done, srv);
         if (!srv->pack) {
             const std::string err = sed_last_error();
             sed_server_destroy(srv);
             return fail(SED_ENOMEM, "packed engine: " + err);
         }
+        srv->fixed = pack_device_bytes(srv->pack);
     }
+    srv->in_use = srv->peak = srv->fixed;
     *out = srv;
     context_child_added(ctx);
     srv->counted = true;
@@ -207,13 +259,15 @@ int sed_server_submit(sed_server* srv, sed_sector* sec, int nev, double tol, uin
     ++srv->n_submitted;
     // Small sectors take a free lane when one is idle (lowest latency); when every lane is
     // busy they join the packed persistent launches at the next check point (throughput).
+    // A sector already running (lane or packed) waits in the queue for its turn.
     bool lane_idle = false;
     if (srv->queue.empty())
         for (const Lane* l : srv->lanes) lane_idle = lane_idle || l->job < 0;
-    if (srv->pack && !lane_idle && pack_eligible(sec, nev)) {
+    if (srv->pack && !lane_idle && !sec->busy && pack_eligible(sec, nev)) {
         r.status = 2;
         r.t_start = r.t_submit;
         r.footprint = pack_footprint(sec, nev, precision);
+        sec->busy = 1;
         auto job = std::make_unique<PackJob>();
         job->id = r.id;
         job->sec = sec;
@@ -231,9 +285,10 @@ int sed_server_submit(sed_server* srv, sed_sector* sec, int nev, double tol, uin
         return SED_OK;
     }
     r.footprint = solve_footprint(sec, nev, precision);
-    if (r.footprint > srv->budget) {
+    if (r.footprint > srv->budget - srv->fixed) {
         r.status = SED_ENOMEM;
-        r.error = "request footprint " + std::to_string(r.footprint) + " exceeds the HBM budget";
+        r.error = "request footprint " + std::to_string(r.footprint) + " B exceeds the server's HBM budget (" +
+                  std::to_string(srv->budget - srv->fixed) + " B after the packed engine's arena)";
         r.t_start = r.t_end = r.t_submit;
         ++srv->n_done;
         srv->reqs[r.id] = r;
@@ -277,6 +332,16 @@ int sed_server_result(sed_server* srv, int64_t request_id, int* status, double* 
         set_error(r.error);
         if (latency_s) *latency_s = std::chrono::duration<double>(r.t_end - r.t_submit).count();
     }
+    return SED_OK;
+}
+
+int sed_server_memory(sed_server* srv, int64_t* budget, int64_t* fixed, int64_t* in_use, int64_t* peak) {
+    if (!srv) return fail(SED_EINVAL, "NULL server");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (budget) *budget = srv->budget;
+    if (fixed) *fixed = srv->fixed;
+    if (in_use) *in_use = srv->in_use;
+    if (peak) *peak = srv->peak;
     return SED_OK;
 }
 

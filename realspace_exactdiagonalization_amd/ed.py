@@ -502,6 +502,13 @@ class SectorServer:
         check(lib().sed_server_result(self._h, rid, C.byref(st), ev, C.byref(stats), C.byref(lat)))
         return st.value, ev, stats.as_dict(), lat.value
 
+    def memory(self) -> dict:
+        """Device-memory accounting: budget, bytes taken at creation (packed engine), bytes
+        charged now and the largest charge so far."""
+        b, f, u, p = (C.c_int64() for _ in range(4))
+        check(lib().sed_server_memory(self._h, C.byref(b), C.byref(f), C.byref(u), C.byref(p)))
+        return {"budget": b.value, "fixed": f.value, "in_use": u.value, "peak": p.value}
+
     def close(self):
         if getattr(self, "_h", None):
             lib().sed_server_destroy(self._h)

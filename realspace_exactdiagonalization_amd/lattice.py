@@ -421,3 +421,49 @@ def workload(name: str) -> Model:
     if name == "haldane3x4":
         return haldane_honeycomb(3, 4, t=1.0, tp=0.6, tpp=-0.58, phi_over_2pi=0.2, n_filled=6)
     raise KeyError(name)
+
+
+# --------------------------------------------------------------------------
+# Serving-stream request families
+# --------------------------------------------------------------------------
+# XXZ chains at half filling, 2 x L Hubbard ladders at quarter filling (U = 4),
+# checkerboard lattices at 1/3 filling of the lower band.
+STREAM_SIZES = {"xxz": [20, 22, 24, 26, 28, 30, 32], "ladder": [6, 7, 8, 9],
+                "checker": [(3, 4), (3, 5), (3, 6), (4, 4), (4, 5), (4, 6)]}
+
+
+def stream_model(family: str, size) -> Model:
+    """Model of one serving-stream request family at one size."""
+    if family == "xxz":
+        return heisenberg_chain(size)
+    if family == "ladder":
+        return hubbard_square(size, 2, U=4.0, t=1.0, n_filled=size)
+    if family == "checker":
+        return checkerboard(size[0], size[1])
+    raise KeyError(family)
+
+
+def stream_requests(n: int, seed: int, sizes=None):
+    """Seeded request list: family and size uniform, momentum draw u in [0, 1) (irrep index
+    floor(u * |G|)), twist uniform in [0, 1)^2 (the chain uses only its x component)."""
+    sizes = STREAM_SIZES if sizes is None else sizes
+    rng = np.random.default_rng(seed)
+    fams = list(sizes)
+    out = []
+    for _ in range(n):
+        f = fams[int(rng.integers(len(fams)))]
+        size = sizes[f][int(rng.integers(len(sizes[f])))]
+        u = float(rng.random())
+        th = (float(rng.random()), float(rng.random()))
+        if f == "xxz":
+            th = (th[0], 0.0)
+        out.append({"family": f, "size": size, "u": u, "theta": th})
+    return out
+
+
+def stream_sector(request):
+    """(model at its twist, group, irrep) of one stream request."""
+    m = twisted(stream_model(request["family"], request["size"]), request["theta"])
+    G = model_group(m, request["theta"])
+    irs = translation_irreps(G, m.lattice)
+    return m, G, irs[min(len(irs) - 1, int(request["u"] * len(irs)))]
