@@ -130,6 +130,16 @@ int sed_matvec(sed_sector* sec, const double* x, double* y, int precision);
  * for FP32) in the sector basis, with or without twisted phases; ordered on the
  * context's stream. */
 int sed_matvec_device(sed_sector* sec, const void* x_dev, void* y_dev, int precision);
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * sed_matvec_device with no allocation and no host synchronisation.  Its scratch (one
+ * vector) is a fixed 64 MiB device array per GPU, set up by sed_create and held until
+ * the process ends, so dim * 16 bytes (FP64) or dim * 8 bytes (FP32) must fit: dim <=
+ * 4,194,304 (FP64) or 8,388,608 (FP32), else SED_EINVAL.  It uses the launch shape an
+ * earlier matvec or solve chose for the sector (shape 1 if none) and never autotunes.
+ * Ordered on the context's stream; calls that overlap on one GPU (other streams,
+ * other contexts) race on the shared scratch array. */
+int sed_matvec_device_noalloc(sed_sector* sec, const void* x_dev, void* y_dev, int precision);
 /* Time n_calls device matvecs on resident random vectors; returns the mean
  * kernel time in ms measured with HIP events on the context's stream. */
 int sed_matvec_benchmark(sed_sector* sec, int precision, int n_calls, double* ms_per_call);

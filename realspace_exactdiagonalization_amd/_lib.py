@@ -77,6 +77,7 @@ SIGNATURES = {
     "sed_sector_hop_counts": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "sed_matvec": (C.c_int, [_vp, _f64p, _f64p, C.c_int]),
     "sed_matvec_device": (C.c_int, [_vp, _vp, _vp, C.c_int]),
+    "sed_matvec_device_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int]),
     "sed_matvec_benchmark": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "sed_sector_set_shape": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "sed_sector_shape": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),

@@ -110,6 +110,9 @@ int sed_create(int device, sed_context** out) {
     }
     ctx->ws = new Workspace();
     ctx->ws->stream = ctx->stream;
+    // the no-allocation matvec's scratch array on this device (its own error is kept for
+    // that entry point; every other entry point works without it)
+    (void)noalloc_scratch_init(device);
     *out = ctx;
     return SED_OK;
 }

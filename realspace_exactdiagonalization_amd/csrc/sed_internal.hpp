@@ -228,6 +228,10 @@ int launch_matvec64(sed_sector* sec, const double* x, double* y, double* xhat, h
 int launch_matvec32(sed_sector* sec, const float* x, float* y, float* xhat, hipStream_t s);
 // gauge D: v <- D^{+-1} v  (forward = multiply by F, backward by conj F)
 int launch_gauge64(sed_sector* sec, double* v, bool forward, hipStream_t s);
+// H|psi> on a prescaled xhat with the sector's chosen launch shape (shape 1 if none),
+// never autotuning; and the per-device scratch array of the no-allocation matvec.
+int launch_matvec_xhat_fixed(sed_sector* sec, const void* xhat, void* y, bool fp64, hipStream_t s);
+int noalloc_scratch_init(int device);
 int launch_gauge32(sed_sector* sec, const float* v, float* out, bool forward, hipStream_t s);
 
 int solve_sector(sed_sector* sec, int nev, double tol, uint64_t seed, int maxiter, int precision,

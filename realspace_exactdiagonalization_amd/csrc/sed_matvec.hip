@@ -533,6 +533,20 @@ int launch_matvec_xhat(sed_sector* sec, const void* xhat, void* y, bool fp64, hi
     return run_mv<float>(sec, reinterpret_cast<const float2*>(xhat), reinterpret_cast<float2*>(y), s);
 }
 
+// The same without the autotuner (no events, no host synchronisation): the shape an
+// earlier matvec or solve chose for this precision, else shape 1 at run length 1.
+int launch_matvec_xhat_fixed(sed_sector* sec, const void* xhat, void* y, bool fp64, hipStream_t s) {
+    if (sec->dim == 0) return SED_OK;
+    const int pi = fp64 ? 0 : 1;
+    const int shape = sec->mv_shape[pi] >= 0 ? sec->mv_shape[pi] : 1;
+    const int chunk = sec->mv_shape[pi] >= 0 ? sec->mv_chunk[pi] : 1;
+    if (fp64)
+        return run_any_shape<double>(sec, shape, chunk, reinterpret_cast<const double2*>(xhat),
+                                     reinterpret_cast<double2*>(y), s);
+    return run_any_shape<float>(sec, shape, chunk, reinterpret_cast<const float2*>(xhat),
+                                reinterpret_cast<float2*>(y), s);
+}
+
 int launch_gauge64(sed_sector* sec, double* v, bool forward, hipStream_t s) {
     if (!sec->has_gauge || sec->dim == 0) return SED_OK;
     const int64_t g = std::min<int64_t>((sec->dim + 255) / 256, 1 << 20);
