@@ -7,12 +7,23 @@ momentum sector k=(0,0) (dim 15,724,214).  One step = one sector matvec
 y = H x with complex-FP32 vectors (FP32 accumulation) over resident device
 buffers; FP64 matvec times are reported beside it.
 
-value = algorithmic bytes of the H|psi> kernel / its mean duration (HIP events
-on the library stream over the timed steps).  Algorithmic bytes per matvec:
+Timed region: the H|psi> kernel alone, launched back to back on the library stream
+over resident buffers, bracketed by HIP events on that stream.  Its input is the
+prescaled vector xhat = sqrt(stab) * x; the prescale pass (D * 2s bytes, 1.3 % of a
+matvec's bytes here) is not timed: inside a Lanczos step it is fused into the kernel
+that normalises the new basis vector.
+
+value = roofline.achieved = algorithmic bytes of the kernel in this library's data
+layout / its mean duration.  Algorithmic bytes per matvec (s = 8 for c32, 16 for c64;
+D, M_g, M counted exactly on the device):
   D * (8 rep mask + 1 stabilizer order + s x[row] + s y[row])
   + M_g * 4   (one 4-byte sector-table entry per gated hop)
-  + M   * s   (one gather of sqrt(stab)*x per in-sector hop)
-with s = 8 (c32) or 16 (c64) and D, M_g, M counted exactly on the device.
+  + M   * s   (one gather of xhat per in-sector hop)
+The workload's own per-unit figures (an 8-byte table entry and a 4-byte norm per row)
+give workload_bytes_per_launch, reported beside it; they count bytes this layout does not
+move, so their rate is not an HBM fraction.  roofline.traffic is the HBM traffic per
+launch measured by rocprofv3 counters (FETCH_SIZE and WRITE_SIZE, copy-calibrated) in
+the committed profile named in roofline.traffic_source.
 """
 import argparse
 import json
@@ -45,19 +56,88 @@ def layout_bytes(dim, gated, ins, s):
     return dim * (8 + 1 + 2 * s) + gated * 4 + ins * s
 
 
+COUNTER_PROFILE = os.path.join("profiles", "matvec_checker4x6_counters.md")
+
+
 def measured_traffic(precision):
-    """HBM bytes per launch of the sector matvec on this workload, from the committed
-    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, scaled by the plain-copy
-    calibration); None when no summary for the current kernel is present."""
-    path = os.path.join(ROOT, "profiles", "matvec_checker4x6_counters.md")
+    """(HBM bytes per launch of the sector matvec on this workload, source) from the
+    committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, scaled by the
+    plain-copy calibration); (None, None) when the summary has no line for it."""
+    path = os.path.join(ROOT, COUNTER_PROFILE)
     if not os.path.exists(path):
-        return None
+        return None, None
     key = f"traffic_bytes_{precision}:"
+    src = None
+    val = None
     with open(path) as f:
         for line in f:
+            if line.strip().startswith("build:"):
+                src = line.split(":", 1)[1].strip()
             if line.strip().startswith(key):
-                return int(line.split(":", 1)[1].strip())
-    return None
+                val = int(line.split(":", 1)[1].strip())
+    if val is None:
+        return None, None
+    return val, f"{COUNTER_PROFILE} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated; {src or 'build unstated'})"
+
+
+def host_info():
+    """CPU model, logical CPUs, CPUs this process may run on, RAM (GiB)."""
+    model, ram = None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal:"):
+                    ram = round(int(line.split()[1]) / 2 ** 20, 1)
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity, "ram_gib": ram,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_solves(threads):
+    """Per-solve CPU times of the oracle's restatement of the reference's matrix-free
+    sector solve (ed_scan_at_irrep_matrixfree!, src/symmetry_resolved_ed.jl:1092-1128):
+    sector basis, serial projection-table build (:680-728), Lanczos eigensolve (ARPACK
+    through scipy over _apply_table!, :731-765, with `threads` per-thread buffers; the
+    reference uses KrylovKit) at tol 1e-10; the catalog is built before the timed region,
+    as the reference's benchmark does (benchmark/benchmark.jl:226-241).  Config[0]'s sector
+    (XXZ N=16, k=0, dim 810, nev 4) and config[3]'s (Haldane 3x4 bosons at zero twist,
+    k=(0,0), dim 11,240), the latter at nev 1 beside the reference's published 0.872 s
+    (benchmark/benchmark_data/benchmark_raw.csv:20-21, hardware unrecorded)."""
+    import numpy as np
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    from oracle import oracle as O
+    from realspace_exactdiagonalization_amd import lattice as L
+    out = {}
+    for name, wl, nev, published in (("xxz16_k0", "xxz16", 4, None), ("haldane3x4_k00", "haldane3x4", 1, 0.872),
+                                     ("haldane3x4_k00_nev3", "haldane3x4", 3, None)):
+        m = L.workload(wl)
+        G = L.model_group(m)
+        ir = L.translation_irreps(G, m.lattice)[0]
+        cat = O.Catalog(m.n_site, m.n_filled, G, m.statistics == L.FERMIONIC)
+        t0 = time.time()
+        sec = O.Sector(cat, ir.values)
+        sec.set_terms(*L.term_arrays(m))
+        n = sec.dim
+        op = LinearOperator((n, n), matvec=lambda x: sec.apply_table(x, threads), dtype=np.complex128)
+        v0 = np.random.default_rng(1234).standard_normal(n) + 0j
+        vals = np.sort(eigsh(op, k=nev, which="SA", tol=1e-10, v0=v0, return_eigenvectors=False).real)
+        dt = time.time() - t0
+        out[name] = {"dim": n, "nev": nev, "s_per_solve": round(dt, 4), "evals": vals.tolist()}
+        if published is not None:
+            out[name]["reference_published_s"] = published
+        log(f"[cpu] solve {name} dim {n} nev {nev}: {dt:.3f} s", flush=True)
+    return out
 
 
 def cpu_baseline(model, G, ir, cat, basis, seconds, threads, bytes_c64, gated, ins):
@@ -101,8 +181,9 @@ def cpu_baseline(model, G, ir, cat, basis, seconds, threads, bytes_c64, gated, i
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": f"on-the-fly matvec (FP64) over columns [0,{cols}) of {D} "
                       f"({100.0 * cols / D:.2f}%), {t_sample:.1f}s; full matvec extrapolated to {t_full:.2f}s "
-                      f"incl. {t_fixed:.2f}s per-thread buffer zero+reduce",
-            "s_per_matvec": round(t_full, 3)}
+                      f"incl. {t_fixed:.2f}s per-thread buffer zero+reduce; rate on the same layout bytes "
+                      f"as the GPU's c64 figure",
+            "s_per_matvec": round(t_full, 3), "host": host_info()}
 
 
 def flux_scan(E, L, ctx, n_twists=16, nev=3, lanes=8):
@@ -448,13 +529,15 @@ def main():
         w0 = time.time()
         ms = H.benchmark(prec, args.steps)
         wall = time.time() - w0
-        B = algorithmic_bytes(dim, gated, ins, s)
-        Bl = layout_bytes(dim, gated, ins, s)
+        B = layout_bytes(dim, gated, ins, s)
+        Bw = algorithmic_bytes(dim, gated, ins, s)
+        traffic, src = measured_traffic(name)
         res[name] = {"ms": ms, "bytes": B, "gbs": B / (ms * 1e-3) / 1e9, "wall_s": wall,
-                     "layout_bytes": Bl, "layout_gbs": Bl / (ms * 1e-3) / 1e9, "shape": basis.shape(prec)}
+                     "workload_bytes": Bw, "workload_gbs": Bw / (ms * 1e-3) / 1e9, "shape": basis.shape(prec),
+                     "traffic": traffic, "traffic_source": src}
         log(f"[bench] {name}: {ms:.3f} ms/matvec  {B/1e9:.2f} GB algorithmic  "
-              f"{res[name]['gbs']:.0f} GB/s ({100*res[name]['gbs']/PEAK_HBM_GBS:.1f}% of 8 TB/s)  "
-              f"shape {res[name]['shape']}", flush=True)
+            f"{res[name]['gbs']:.0f} GB/s ({100*res[name]['gbs']/PEAK_HBM_GBS:.1f}% of 8 TB/s)  "
+            f"shape {res[name]['shape']}", flush=True)
 
     flux = None
     if not args.no_flux:
@@ -473,9 +556,19 @@ def main():
               f"{stream['server_fp32_refine']['p50_ms']}/{stream['server_fp32_refine']['p99_ms']} ms "
               f"(plain {stream['plain_fp64']['p50_ms']}/{stream['plain_fp64']['p99_ms']} ms)", flush=True)
     main_r = res["c32"]
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    cpu = None if args.no_cpu else cpu_baseline(model, G, ir, cat, basis, args.cpu_seconds, threads,
-                                                res["c64"]["bytes"], gated, ins)
+    # the host cores this process may use (the harness's share on a shared box: OMP_NUM_THREADS)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(model, G, ir, cat, basis, args.cpu_seconds, threads, res["c64"]["bytes"], gated, ins)
+        cpu["per_solve"] = cpu_solves(threads)
+
+    def roof(r):
+        return {"bound": "hbm", "achieved": round(r["gbs"], 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(r["gbs"] / PEAK_HBM_GBS, 4), "traffic": r["traffic"],
+                "traffic_source": r["traffic_source"],
+                "traffic_frac": None if r["traffic"] is None else round(r["traffic"] / (r["ms"] * 1e-3) / 1e9 /
+                                                                      PEAK_HBM_GBS, 4)}
     out = {
         "metric": "sector matvec GB/s (% HBM peak)",
         "value": round(main_r["gbs"], 2),
@@ -492,20 +585,22 @@ def main():
         "config": {"workload": f"{args.workload} k=(0,0) sector matvec", "sector_dim": dim,
                    "n_site": model.n_site, "n_filled": model.n_filled, "group_order": G.order,
                    "gated_hops": gated, "in_sector_hops": ins},
-        "roofline": {"bound": "hbm", "achieved": round(main_r["gbs"], 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(main_r["gbs"] / PEAK_HBM_GBS, 4), "traffic": measured_traffic("c32")},
+        "roofline": roof(main_r),
         "algorithmic_bytes_per_launch": main_r["bytes"],
-        "layout_bytes": {"c32": res["c32"]["layout_bytes"], "c64": res["c64"]["layout_bytes"],
-                         "c32_GB/s": round(res["c32"]["layout_gbs"], 2), "c64_GB/s": round(res["c64"]["layout_gbs"], 2)},
+        "workload_bytes_per_launch": {"c32": res["c32"]["workload_bytes"], "c64": res["c64"]["workload_bytes"],
+                                      "c32_GB/s": round(res["c32"]["workload_gbs"], 2),
+                                      "c64_GB/s": round(res["c64"]["workload_gbs"], 2)},
         "cpu_baseline": cpu,
-        "fp64": {"ms_per_matvec": round(res["c64"]["ms"], 4), "GB/s": round(res["c64"]["gbs"], 2),
-                 "frac": round(res["c64"]["gbs"] / PEAK_HBM_GBS, 4), "traffic": measured_traffic("c64"),
-                 "launch_shape": res["c64"]["shape"]},
+        "fp64": {"ms_per_matvec": round(res["c64"]["ms"], 4), "algorithmic_bytes_per_launch": res["c64"]["bytes"],
+                 "roofline": roof(res["c64"]), "launch_shape": res["c64"]["shape"]},
         "launch_shape": res["c32"]["shape"],
         "flux_scan_haldane3x4": flux,
         "serving_stream": stream,
         "setup_s": {"catalog": round(t_cat, 3), "sector": round(t_sec, 3), "total": round(time.time() - t_setup, 3)},
     }
+    basis.close()
+    cat.close()
+    ctx.close()
     print(json.dumps(out), flush=True)
 
 

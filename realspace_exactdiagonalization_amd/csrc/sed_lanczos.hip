@@ -474,7 +474,16 @@ static int pro_max_nev() {
     return e && *e ? std::atoi(e) : kProMaxNev;
 }
 
+// Captured steps are replayed unless SED_LANCZOS_GRAPH=0, and never under rocprofv3's
+// kernel tracing (it sets ROCPROF_KERNEL_TRACE): traced runs whose solve launched a step
+// graph after an earlier solve in the process (the 48-site checkerboard sector solved
+// twice; a flux scan of 24-site sectors) ended with SIGSEGV in a host memcpy from an
+// unmapped address, called from the profiler's library below hipGraphLaunch.  The same
+// solves without the profiler, and eager launches of the same kernels under it, run
+// without fault.
 static bool graphs_enabled() {
+    const char* t = std::getenv("ROCPROF_KERNEL_TRACE");
+    if (t && *t && t[0] != '0') return false;
     const char* e = std::getenv("SED_LANCZOS_GRAPH");
     return !(e && e[0] == '0');
 }
