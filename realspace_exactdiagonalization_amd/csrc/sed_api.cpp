@@ -9,6 +9,7 @@
 #include <utility>
 
 #include "sed_internal.hpp"
+#include "sed_mv.hpp"
 
 namespace sed {
 
@@ -385,6 +386,21 @@ int sed_sector_build(sed_catalog* cat, const double* chi, double atol, sed_secto
     for (int i = 0; i < 2 * cat->n_group; ++i) c32[i] = (float)chi[i];
     SED_TRY(upload(sec->d_chi32, c32.data(), c32.size(), s));
     SED_TRY(launch_sector(sec, sec->d_chi64.p));
+    {
+        // characters with the hop sign folded in, indexed like a table entry's low bits
+        // (g << 1 | sign), and a zero entry for hops that do not land in the sector
+        const int nc = (1 << sec->shift) + 1;
+        std::vector<double> c2(2 * (size_t)nc, 0.0);
+        for (int g = 0; g < cat->n_group && (2 * g + 1) < nc - 1; ++g)
+            for (int sg = 0; sg < 2; ++sg) {
+                c2[2 * (2 * g + sg)] = sg ? -chi[2 * g] : chi[2 * g];
+                c2[2 * (2 * g + sg) + 1] = sg ? -chi[2 * g + 1] : chi[2 * g + 1];
+            }
+        std::vector<float> c2f(c2.begin(), c2.end());
+        SED_TRY(upload(sec->d_chi2_64, c2.data(), c2.size(), s));
+        SED_TRY(upload(sec->d_chi2_32, c2f.data(), c2f.size(), s));
+        SED_HIP(hipStreamSynchronize(s));   // the host arrays end with this scope
+    }
     sec->has_gauge = cat->has_phases;
     sec->gauge_f = cat->gauge_f;
     sec->phases = cat->phases;
@@ -496,6 +512,35 @@ int sed_sector_set_terms(sed_sector* sec, int n_bil, const int32_t* i_from, cons
     SED_TRY(upload(sec->d_dens_v64, dv64.data(), dv64.size(), s));
     SED_TRY(upload(sec->d_term_coef32, tc32.data(), tc32.size(), s));
     SED_TRY(upload(sec->d_dens_v32, dv32.data(), dv32.size(), s));
+    // padded copy: each site's list to a multiple of kTermPad, padding hops a -> a with 0
+    {
+        std::vector<int32_t> ps(N + 1, 0);
+        std::vector<uint32_t> info;
+        std::vector<double> pc64;
+        for (int a = 0; a < N; ++a) {
+            ps[a] = (int32_t)info.size();
+            const int t0 = sec->term_start[a], t1 = sec->term_start[a + 1];
+            const int cnt = t1 - t0, padded = (cnt + kTermPad - 1) / kTermPad * kTermPad;
+            for (int i = 0; i < padded; ++i) {
+                const int b = i < cnt ? sec->term_to[t0 + i] : a;
+                info.push_back((uint32_t)b | ((uint32_t)(b * cat->bstride) << 8));
+                pc64.push_back(i < cnt ? tc64[2 * (t0 + i)] : 0.0);
+                pc64.push_back(i < cnt ? tc64[2 * (t0 + i) + 1] : 0.0);
+            }
+        }
+        ps[N] = (int32_t)info.size();
+        sec->n_pterms = (int)info.size();
+        if (info.empty()) {
+            info.push_back(0);
+            pc64.assign(2, 0.0);
+        }
+        std::vector<float> pc32(pc64.begin(), pc64.end());
+        SED_TRY(upload(sec->d_pterm_start, ps.data(), ps.size(), s));
+        SED_TRY(upload(sec->d_pterm_info, info.data(), info.size(), s));
+        SED_TRY(upload(sec->d_pterm_coef64, pc64.data(), pc64.size(), s));
+        SED_TRY(upload(sec->d_pterm_coef32, pc32.data(), pc32.size(), s));
+        SED_HIP(hipStreamSynchronize(s));   // the host arrays end with this scope
+    }
     SED_HIP(hipStreamSynchronize(s));
     const size_t need = std::max(matvec_lds_bytes<double>(sec), matvec_lds_bytes<float>(sec));
     if ((int)need > max_lds(cat->ctx))
@@ -781,6 +826,8 @@ int sed_sector_one_body(sed_sector* sec, int which, double* rho) {
     tmp.d_table.borrow(sec->d_table);
     tmp.d_chi64.borrow(sec->d_chi64);
     tmp.d_chi32.borrow(sec->d_chi32);
+    tmp.d_chi2_64.borrow(sec->d_chi2_64);
+    tmp.d_chi2_32.borrow(sec->d_chi2_32);
     tmp.has_gauge = sec->has_gauge;
     tmp.d_gauge.borrow(sec->d_gauge);
     tmp.gauge_f = sec->gauge_f;

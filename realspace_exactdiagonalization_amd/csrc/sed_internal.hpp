@@ -152,6 +152,14 @@ struct sed_sector {
     sed::DevBuf<double> d_term_coef64, d_dens_v64, d_chi64;  // interleaved complex
     sed::DevBuf<float> d_term_coef32, d_dens_v32, d_chi32;
     sed::DevBuf<double> d_gauge;      // F(rep_r) per row, complex interleaved (only if has_gauge)
+    // the term lists padded per source site to a multiple of kTermPad (site-by-site hop loop)
+    int n_pterms = 0;
+    sed::DevBuf<int32_t> d_pterm_start;
+    sed::DevBuf<uint32_t> d_pterm_info;    // target b | (b * binomial stride) << 8
+    sed::DevBuf<double> d_pterm_coef64;
+    sed::DevBuf<float> d_pterm_coef32;
+    sed::DevBuf<double> d_chi2_64;         // (-1)^p chi(g) at g << 1 | p, then one 0 entry
+    sed::DevBuf<float> d_chi2_32;
     int64_t hops_gated = -1, hops_in_sector = -1;
     int mv_shape[2] = {-1, -1};       // H|psi> launch shape per precision (FP64, FP32), -1 = not tuned
     int mv_chunk[2] = {1, 1};         // XCD run length of the H|psi> blocks (0 = one range per XCD)

@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(BLOCK) k_matvec(MvConst c, const uint64_t* __r
                                                   int64_t chunk) {
     using T = typename C2<R>::T;
     constexpr int RPB = BLOCK / LPR;
-    const MvLds<R> L = stage_mv<R>(c, BLOCK, FLAT);
+    const MvLds<R> L = stage_mv<R>(c, BLOCK, FLAT, !FLAT);
     const int64_t lb = xcd_block(nblocks, chunk);
     if (lb < 0) return;
     const int sub = threadIdx.x % LPR;
@@ -61,10 +61,11 @@ __global__ void __launch_bounds__(BLOCK) k_matvec(MvConst c, const uint64_t* __r
     const bool live = r < c.dim;
     const M m = live ? (M)rep[r] : (M)0;
     R ax = 0, ay = 0, dx = 0, dy = 0;
-    if (FLAT)
+    if constexpr (FLAT)
         row_sums_flat<M, R, FERMI, LPR, BATCH, BLOCK>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx, dy);
     else
-        row_sums<M, R, FERMI, LPR, BATCH, BLOCK>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx, dy);
+        row_sums_padded<M, R, FERMI, LPR, BATCH, BLOCK>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx,
+                                                        dy);
     if (LPR > 1) {
 #pragma unroll
         for (int off = LPR / 2; off >= 1; off >>= 1) {
@@ -285,6 +286,16 @@ static MvConst make_const(const sed_sector* sec) {
     c.shift = sec->shift;
     c.n_terms = sec->n_terms;
     c.n_dens = sec->n_dens;
+    c.n_pterms = sec->n_pterms;
+    c.pstart = sec->d_pterm_start.p;
+    c.pinfo = sec->d_pterm_info.p;
+    if (sizeof(R) == 8) {
+        c.pcoef = sec->d_pterm_coef64.p;
+        c.chi2 = sec->d_chi2_64.p;
+    } else {
+        c.pcoef = sec->d_pterm_coef32.p;
+        c.chi2 = sec->d_chi2_32.p;
+    }
     c.gmask = (1u << (sec->shift - 1)) - 1u;
     c.dim = sec->dim;
     return c;
@@ -296,7 +307,7 @@ template <typename R>
 size_t matvec_lds_bytes(const sed_sector* sec) {
     MvConst c = make_const<R>(sec);
     size_t off[9];
-    return mv_lds_layout<R>(c, off, kMvBlock);
+    return std::max(mv_lds_layout<R>(c, off, kMvBlock), mv_lds_layout<R>(c, off, kMvBlock, false, true));
 }
 template size_t matvec_lds_bytes<float>(const sed_sector*);
 template size_t matvec_lds_bytes<double>(const sed_sector*);
@@ -313,7 +324,7 @@ template <typename M, typename R, bool F, int LPR, int B, int BLOCK, bool FLAT =
 static int launch_shape(const MvConst& c, sed_sector* sec, const typename C2<R>::T* xh, typename C2<R>::T* y,
                         int chunk, hipStream_t s) {
     size_t off[9];
-    const size_t lds = mv_lds_layout<R>(c, off, BLOCK, FLAT);
+    const size_t lds = mv_lds_layout<R>(c, off, BLOCK, FLAT, !FLAT);
     if ((int)lds > sec->cat->ctx->max_lds) return fail(SED_EUNSUPPORTED, "launch shape needs too much LDS");
     if (FLAT && !flat_ranks_exact(sec)) return fail(SED_EUNSUPPORTED, "flat hop loop: C(n-1, k+1) >= 2^32");
     const int64_t rpb = BLOCK / LPR;
@@ -353,7 +364,8 @@ static bool shape_fits(const sed_sector* sec, int shape) {
     MvConst c = make_const<R>(sec);
     size_t off[9];
     if (kShapes[shape].flat && !flat_ranks_exact(sec)) return false;
-    return (int)mv_lds_layout<R>(c, off, kShapes[shape].block, kShapes[shape].flat) <= sec->cat->ctx->max_lds;
+    return (int)mv_lds_layout<R>(c, off, kShapes[shape].block, kShapes[shape].flat, !kShapes[shape].flat) <=
+           sec->cat->ctx->max_lds;
 }
 
 template <typename R>

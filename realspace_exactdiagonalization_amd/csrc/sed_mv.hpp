@@ -7,6 +7,9 @@
 namespace sed {
 
 constexpr int kMvBlock = 256;
+// Each source site's term list in the padded copy is a multiple of kTermPad long; padding
+// terms hop from the site to itself (always gated out: the site is occupied) with t' = 0.
+constexpr int kTermPad = 4;
 
 struct MvConst {
     const uint32_t* bin;
@@ -17,11 +20,21 @@ struct MvConst {
     const uint8_t* dj;
     const void* dv;
     const void* chi;
-    int n, k, nG, bs, shift, n_terms, n_dens;
+    // padded term lists: per entry b | (b * bs) << 8, conj(t'); per-site starts; and the
+    // characters with the sign folded in: chi2[g << 1 | p] = (-1)^p chi(g) for index
+    // g << 1 | p < 1 << shift, chi2[1 << shift] = 0 (dead hops)
+    const int32_t* pstart;
+    const uint32_t* pinfo;
+    const void* pcoef;
+    const void* chi2;
+    int n, k, nG, bs, shift, n_terms, n_dens, n_pterms;
     uint32_t gmask;
     int64_t dim;
 };
 
+// Constants of the hop loops in LDS.  With the padded term lists (`padded` layouts),
+// tstart / tcoef / chi hold the padded starts, coefficients and sign-folded characters,
+// pinfo the packed targets, and tto is unused.
 template <typename R>
 struct MvLds {
     using T = typename C2<R>::T;
@@ -32,6 +45,7 @@ struct MvLds {
     T* dv;
     T* chi;
     uint8_t* tto;
+    uint32_t* pinfo;
     uint8_t* dj;
     uint32_t* S;
 };
@@ -44,28 +58,33 @@ __host__ __device__ inline int mv_scratch(const MvConst& c, bool flat) {
     return flat ? 2 * (c.k + 1) : (c.k > 0 ? c.k : 1);
 }
 
+__host__ __device__ inline int mv_chi2_count(const MvConst& c) { return (1 << c.shift) + 1; }
+
 template <typename R>
-__host__ __device__ inline size_t mv_lds_layout(const MvConst& c, size_t* off, int block, bool flat = false) {
+__host__ __device__ inline size_t mv_lds_layout(const MvConst& c, size_t* off, int block, bool flat = false,
+                                                bool padded = false) {
     using T = typename C2<R>::T;
+    const size_t nt = padded ? (size_t)c.n_pterms : (size_t)c.n_terms;
+    const size_t nchi = padded ? (size_t)mv_chi2_count(c) : (size_t)c.nG;
     size_t o = 0;
     off[0] = o; o = align16(o + (size_t)(c.n + 1) * c.bs * 4);       // bin
     off[1] = o; o = align16(o + (size_t)(c.n + 1) * 4);               // tstart
     off[2] = o; o = align16(o + (size_t)(c.n + 1) * 4);               // dstart
-    off[3] = o; o = align16(o + (size_t)c.n_terms * sizeof(T));       // tcoef
+    off[3] = o; o = align16(o + nt * sizeof(T));                      // tcoef
     off[4] = o; o = align16(o + (size_t)c.n_dens * sizeof(T));        // dv
-    off[5] = o; o = align16(o + (size_t)c.nG * sizeof(T));            // chi
-    off[6] = o; o = align16(o + (size_t)c.n_terms);                   // tto
+    off[5] = o; o = align16(o + nchi * sizeof(T));                    // chi
+    off[6] = o; o = align16(o + nt * (padded ? 4 : 1));               // tto / pinfo
     off[7] = o; o = align16(o + (size_t)c.n_dens);                    // dj
     off[8] = o; o = align16(o + (size_t)mv_scratch(c, flat) * block * 4);   // S
     return o;
 }
 
 template <typename R>
-__device__ __forceinline__ MvLds<R> stage_mv(const MvConst& c, int block, bool flat = false) {
+__device__ __forceinline__ MvLds<R> stage_mv(const MvConst& c, int block, bool flat = false, bool padded = false) {
     using T = typename C2<R>::T;
     extern __shared__ __align__(16) uint8_t smem[];
     size_t off[9];
-    mv_lds_layout<R>(c, off, block, flat);
+    mv_lds_layout<R>(c, off, block, flat, padded);
     MvLds<R> L;
     L.bin = reinterpret_cast<uint32_t*>(smem + off[0]);
     L.tstart = reinterpret_cast<int32_t*>(smem + off[1]);
@@ -74,26 +93,38 @@ __device__ __forceinline__ MvLds<R> stage_mv(const MvConst& c, int block, bool f
     L.dv = reinterpret_cast<T*>(smem + off[4]);
     L.chi = reinterpret_cast<T*>(smem + off[5]);
     L.tto = smem + off[6];
+    L.pinfo = reinterpret_cast<uint32_t*>(smem + off[6]);
     L.dj = smem + off[7];
     L.S = reinterpret_cast<uint32_t*>(smem + off[8]);
     const int nt = blockDim.x;
     for (int i = threadIdx.x; i < (c.n + 1) * c.bs; i += nt) L.bin[i] = c.bin[i];
+    const int32_t* ts = padded ? c.pstart : c.tstart;
     for (int i = threadIdx.x; i <= c.n; i += nt) {
-        L.tstart[i] = c.tstart[i];
+        L.tstart[i] = ts[i];
         L.dstart[i] = c.dstart[i];
     }
-    const T* tc = reinterpret_cast<const T*>(c.tcoef);
     const T* dv = reinterpret_cast<const T*>(c.dv);
-    const T* ch = reinterpret_cast<const T*>(c.chi);
-    for (int i = threadIdx.x; i < c.n_terms; i += nt) {
-        L.tcoef[i] = tc[i];
-        L.tto[i] = c.tto[i];
+    if (padded) {
+        const T* tc = reinterpret_cast<const T*>(c.pcoef);
+        const T* ch = reinterpret_cast<const T*>(c.chi2);
+        for (int i = threadIdx.x; i < c.n_pterms; i += nt) {
+            L.tcoef[i] = tc[i];
+            L.pinfo[i] = c.pinfo[i];
+        }
+        for (int i = threadIdx.x; i < mv_chi2_count(c); i += nt) L.chi[i] = ch[i];
+    } else {
+        const T* tc = reinterpret_cast<const T*>(c.tcoef);
+        const T* ch = reinterpret_cast<const T*>(c.chi);
+        for (int i = threadIdx.x; i < c.n_terms; i += nt) {
+            L.tcoef[i] = tc[i];
+            L.tto[i] = c.tto[i];
+        }
+        for (int i = threadIdx.x; i < c.nG; i += nt) L.chi[i] = ch[i];
     }
     for (int i = threadIdx.x; i < c.n_dens; i += nt) {
         L.dv[i] = dv[i];
         L.dj[i] = c.dj[i];
     }
-    for (int i = threadIdx.x; i < c.nG; i += nt) L.chi[i] = ch[i];
     __syncthreads();
     return L;
 }
@@ -182,6 +213,88 @@ __device__ __forceinline__ void row_sums(const MvLds<R>& L, const MvConst& c, co
                 const R sg = live ? ((((e[u] & 1) ^ (uint32_t)pj[u]) != 0) ? (R)-1 : (R)1) : (R)0;
                 ax += sg * pr.x;
                 ay += sg * pr.y;
+            }
+        }
+        ia += LPR;
+    }
+}
+
+// row_sums over the padded term lists (stage_mv with padded = true): every site's list is
+// a multiple of kTermPad long, so a batch needs no bounds test, each entry carries its
+// target's binomial row offset, and the hop's sign (table parity xor Jordan-Wigner parity)
+// selects the signed character chi2[g << 1 | sign]; hops that are gated out or leave the
+// sector select chi2[1 << shift] = 0.  Each hop then adds conj(t') chi2 xhat[c] with four
+// fused multiply-adds.
+template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST>
+__device__ __forceinline__ void row_sums_padded(const MvLds<R>& L, const MvConst& c, const M m, const int sub,
+                                                uint32_t* S, const uint32_t* __restrict__ table,
+                                                const typename C2<R>::T* __restrict__ xhat, R& ax, R& ay, R& dx,
+                                                R& dy) {
+    using T = typename C2<R>::T;
+    static_assert(kTermPad % BATCH == 0, "batch must divide the term padding");
+    const uint32_t mask2 = (1u << c.shift) - 1u;
+    const uint32_t dead = mask2 + 1u;
+    M rest = m;
+    for (int i = 0; i < sub && rest; ++i) rest &= rest - 1;
+    int ia = sub;
+    while (rest) {
+        const int a = ctz_m<M>(rest);
+        for (int i = 0; i < LPR && rest; ++i) rest &= rest - 1;
+        const M mp = m ^ ((M)1 << a);
+        uint32_t base = 0;
+        {
+            uint32_t cur = 0;
+            S[0] = 0;
+            M t = mp;
+            int j = 0;
+            while (t) {
+                const int q = ctz_m<M>(t);
+                t &= t - 1;
+                const uint32_t* bq = L.bin + __umul24(q, c.bs) + j;
+                const uint32_t c1 = bq[1], c2 = bq[2];
+                base += c2;
+                cur += c1 - c2;
+                ++j;
+                S[j * SST] = cur;
+            }
+        }
+        for (int d = L.dstart[a]; d < L.dstart[a + 1]; ++d) {
+            if ((m >> L.dj[d]) & 1) {
+                dx += L.dv[d].x;
+                dy += L.dv[d].y;
+            }
+        }
+        const int t1 = L.tstart[a + 1];
+        for (int t = L.tstart[a]; t < t1; t += BATCH) {
+            uint32_t idx[BATCH];
+            bool ok[BATCH];
+            uint32_t pj[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const uint32_t info = L.pinfo[t + u];
+                const int b = (int)(info & 63u);
+                const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
+                ok[u] = !((m >> b) & 1);
+                idx[u] = base + S[ib * SST] + L.bin[(info >> 8) + ib + 1];
+                pj[u] = FERMI ? (uint32_t)((ib ^ ia) & 1) : 0u;
+            }
+            uint32_t e[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const uint32_t v = table[ok[u] ? idx[u] : 0u];
+                e[u] = ok[u] ? v : kNoRow;
+            }
+            T v[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) v[u] = xhat[e[u] != kNoRow ? (e[u] >> c.shift) : 0u];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const uint32_t ci = e[u] != kNoRow ? ((e[u] ^ pj[u]) & mask2) : dead;
+                const T cf = cmul(L.tcoef[t + u], L.chi[ci]);
+                ax += cf.x * v[u].x;
+                ax -= cf.y * v[u].y;
+                ay += cf.x * v[u].y;
+                ay += cf.y * v[u].x;
             }
         }
         ia += LPR;
