@@ -164,7 +164,7 @@ struct sed_sector {
     int mv_shape[2] = {-1, -1};       // H|psi> launch shape per precision (FP64, FP32), -1 = not tuned
     int mv_chunk[2] = {1, 1};         // XCD run length of the H|psi> blocks (0 = one range per XCD)
     int64_t version = 0;              // bumped whenever device term tables are (re)uploaded
-    float tune_ms[2][32] = {};        // autotuner timings: shapes, XCD run lengths, shapes again
+    float tune_ms[2][64] = {};        // autotuner timings: shapes, XCD run lengths, shapes again
     int busy = 0;                     // a server request is running on this sector
     // last solve
     std::vector<double> last_evals;

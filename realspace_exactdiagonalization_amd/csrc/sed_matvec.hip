@@ -103,6 +103,7 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // XCD run lengths the autotuner tries for the chosen shape (0 = one range per XCD)
 static const int kChunks[] = {1, 16, 64, 256, 1024, 0};
 constexpr int kNumChunks = sizeof(kChunks) / sizeof(kChunks[0]);
+static_assert(2 * kNumShapes + kNumChunks <= 64, "autotuner timings must fit sed_sector::tune_ms");
 
 // Dense H' (row-major, dim x dim, complex double) for the small-sector path.
 template <typename M, bool FERMI>

@@ -20,14 +20,18 @@ Every call runs on the GPU through the HIP library; there is no CPU path here.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
+from fractions import Fraction
 from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
 from . import _lib
+from . import checkpoint as _ckpt
 from ._lib import PREC_FP32, PREC_FP32_REFINE, PREC_FP64, SolveStats, check, lib, ptr
-from .lattice import FERMIONIC, Irrep, Model, SymmetryGroup, model_group, term_arrays, translation_irreps, twisted
+from .lattice import (FERMIONIC, Irrep, Model, SymmetryGroup, identity_group, model_group, term_arrays,
+                      translation_irreps, twisted)
 
 __all__ = [
     "Context", "SymmetryOrbitCatalog", "SymmetrySectorBasis", "MatrixFreeHamiltonian",
@@ -202,7 +206,7 @@ class SymmetrySectorBasis:
 
     def shape(self, precision: int) -> dict:
         sh, ln, bt, ch, ns, nc, wg = (C.c_int() for _ in range(7))
-        ms = np.zeros(32, dtype=np.float32)
+        ms = np.zeros(64, dtype=np.float32)
         check(lib().sed_sector_shape(self._h, precision, C.byref(sh), C.byref(ln), C.byref(bt), C.byref(ch),
                                      C.byref(ns), C.byref(nc), ptr(ms), C.byref(wg)))
         return {"shape": sh.value, "lanes_per_row": ln.value, "batch": bt.value, "workgroup": wg.value,
@@ -374,22 +378,32 @@ def off_diagonal_long_range_order(H_op: MatrixFreeHamiltonian, which: int = 0) -
 
 @dataclass
 class EDData:
-    """``Symmetry_Resolved_ED_Data`` (``src/symmetry_resolved_ed.jl:1051-1060``)."""
+    """``Symmetry_Resolved_ED_Data`` (``src/symmetry_resolved_ed.jl:1051-1060``).  The
+    catalog is None only for scans driven by a caller-supplied sector solver."""
     model: Model
     n_filled: int
     symmetry_group: SymmetryGroup
     irrep_list: List[Irrep]
-    orbit_catalog: SymmetryOrbitCatalog
+    orbit_catalog: Optional[SymmetryOrbitCatalog]
     sector_dims: List[int]
     ed_scan_res: Dict[int, Tuple[np.ndarray, np.ndarray]] = field(default_factory=dict)
+    checkpoint_path: Optional[str] = None
+
+    @property
+    def filling_fraction(self) -> Fraction:
+        """Particles per unit cell (the reference's filling_fraction)."""
+        L1, L2 = self.model.lattice.sample_size
+        return Fraction(self.n_filled, L1 * L2)
 
 
 def build_ed_data(model: Model, symmetry_group: Optional[SymmetryGroup] = None,
-                  ctx: Optional[Context] = None) -> EDData:
+                  ctx: Optional[Context] = None, build_catalog: bool = True) -> EDData:
     G = symmetry_group if symmetry_group is not None else model_group(model)
     irreps = translation_irreps(G, model.lattice)
-    cat = build_symmetry_orbit_catalog(n_site=model.n_site, n_filled=model.n_filled, symmetry_group=G,
-                                       particle_statistics=model.statistics, ctx=ctx)
+    cat = None
+    if build_catalog:
+        cat = build_symmetry_orbit_catalog(n_site=model.n_site, n_filled=model.n_filled, symmetry_group=G,
+                                           particle_statistics=model.statistics, ctx=ctx)
     return EDData(model, model.n_filled, G, irreps, cat, [0] * len(irreps))
 
 
@@ -417,9 +431,33 @@ def ed_scan_at_irrep_matrix(irrep_label, ed_data: EDData, nev: int = 5, **solve_
     return ed_scan_at_irrep_matrixfree(irrep_label, ed_data, nev=nev, **solve_kw)
 
 
-def _scan_sectors(ed_data: EDData, nev: int, mode: str, scanned_sectors, **solve_kw) -> EDData:
+def _state(ed_data: EDData) -> _ckpt.ScanState:
+    return _ckpt.ScanState(ed_data.model, ed_data.n_filled, ed_data.filling_fraction, ed_data.symmetry_group.name,
+                           [tuple(ir.label) for ir in ed_data.irrep_list], ed_data.sector_dims, ed_data.ed_scan_res)
+
+
+def _scan_sectors(ed_data: EDData, nev: int, mode: str, scanned_sectors, checkpoint_path: Optional[str] = None,
+                  overwrite: bool = False, _checkpoint_loaded: bool = False, solve=None, **solve_kw) -> EDData:
+    """``_ed_scan_sectors!`` (``:1199-1256``).  With ``checkpoint_path``: unless
+    ``overwrite``, an existing checkpoint is loaded first (it must be of the same filling,
+    sector labels and Hamiltonian, or the scan stops with an error) and its finished
+    sectors are skipped; after each solved sector the checkpoint is rewritten atomically.
+    ``solve(ed_data, irrep_index, nev) -> (vals, vecs, dim)`` replaces the GPU sector solve
+    (for drivers and tests that bring their own solver)."""
     if mode not in ("matrix", "matrixfree"):
         raise ValueError(f"Unknown ED scan mode: {mode}. Use 'matrix' or 'matrixfree'.")
+    if overwrite:
+        ed_data.ed_scan_res.clear()
+    elif checkpoint_path is not None and os.path.isfile(checkpoint_path) and not _checkpoint_loaded:
+        st = _ckpt.load_checkpoint(checkpoint_path)
+        if (st.n_filled != ed_data.n_filled or st.filling_fraction != ed_data.filling_fraction
+                or st.irrep_labels != [tuple(ir.label) for ir in ed_data.irrep_list]
+                or not _ckpt.same_flux_problem(st.model, ed_data.model)):
+            raise RuntimeError("ED checkpoint is incompatible with the requested filling or symmetry-sector labels.")
+        ed_data.ed_scan_res = st.ed_scan_res
+        ed_data.sector_dims = list(st.sector_dims)
+    if checkpoint_path is not None:
+        ed_data.checkpoint_path = checkpoint_path
     wanted = None if scanned_sectors is None else {tuple(l) for l in scanned_sectors}
     for idx, ir in enumerate(ed_data.irrep_list):
         if wanted is not None and tuple(ir.label) not in wanted:
@@ -428,16 +466,43 @@ def _scan_sectors(ed_data: EDData, nev: int, mode: str, scanned_sectors, **solve
             dim = ed_data.sector_dims[idx]
             if dim == 0 or len(ed_data.ed_scan_res[idx][0]) >= min(nev, dim):
                 continue
-        if mode == "matrixfree":
+        if solve is not None:
+            vals, vecs, dim = solve(ed_data, idx, nev)
+            ed_data.sector_dims[idx] = int(dim)
+            ed_data.ed_scan_res[idx] = (np.asarray(vals), np.asarray(vecs))
+        elif mode == "matrixfree":
             ed_scan_at_irrep_matrixfree(ir.label, ed_data, nev=nev, **solve_kw)
         else:
             ed_scan_at_irrep_matrix(ir.label, ed_data, nev=nev, **solve_kw)
+        if checkpoint_path is not None:
+            _ckpt.save_checkpoint(_state(ed_data), checkpoint_path)
     return ed_data
+
+
+def resume_ed_scan(checkpoint_path: str, nev: int = 5, mode: str = "matrix", scanned_sectors=None,
+                   overwrite: bool = False, ctx: Optional[Context] = None, solve=None, **solve_kw) -> EDData:
+    """``resume_ed_scan!`` (``:1266-1277``): the scan state is read from the checkpoint
+    (model, filling, group, sectors finished so far), the orbit catalog rebuilt, and the
+    missing or undersized sectors solved with per-sector checkpoints."""
+    if not os.path.isfile(checkpoint_path):
+        raise RuntimeError(f"ED checkpoint does not exist: {checkpoint_path}")
+    st = _ckpt.load_checkpoint(checkpoint_path)
+    m = st.model
+    m.n_filled = st.n_filled
+    G = identity_group(m.n_site) if st.group_name == "identity" else model_group(m, m.theta)
+    d = build_ed_data(m, symmetry_group=G, ctx=ctx, build_catalog=solve is None)
+    if [tuple(ir.label) for ir in d.irrep_list] != st.irrep_labels:
+        raise RuntimeError("ED checkpoint is incompatible with the requested filling or symmetry-sector labels.")
+    d.ed_scan_res = st.ed_scan_res
+    d.sector_dims = list(st.sector_dims)
+    return _scan_sectors(d, nev, mode, scanned_sectors, checkpoint_path=checkpoint_path, overwrite=overwrite,
+                         _checkpoint_loaded=True, solve=solve, **solve_kw)
 
 
 def ed_scan(ed_data: EDData, nev: int = 5, mode: str = "matrix", scanned_sectors=None, *, flux_direction: int = 1,
             twisted_phases_over_2pi_list: Optional[Sequence[float]] = None, ctx: Optional[Context] = None,
-            **solve_kw):
+            checkpoint_path: Optional[str] = None, checkpoint_dir: Optional[str] = None, overwrite: bool = False,
+            solve=None, **solve_kw):
     """``ed_scan!`` (``:1297-1356``, sector loop ``_ed_scan_sectors!`` ``:1199-1256``).
 
     Without a flux list: every momentum sector of ``ed_data`` (or those whose labels are in
@@ -446,27 +511,46 @@ def ed_scan(ed_data: EDData, nev: int = 5, mode: str = "matrix", scanned_sectors
     Returns ``ed_data``.  With ``twisted_phases_over_2pi_list``: for each twist theta along
     ``flux_direction`` (1-based) the model is re-twisted, the flux-aware translation group
     (or the identity group when ``ed_data`` uses it) and a fresh orbit catalog built, and the
-    sectors scanned; returns the list of per-twist ``EDData`` (the reference writes one
-    checkpoint per twist and returns their paths; checkpoint files are not written here).
+    sectors scanned; returns the list of per-twist ``EDData``.
     ``mode`` is ``'matrix'`` (the reference's default) or ``'matrixfree'``; both solve on the
-    GPU (``ed_scan_at_irrep_matrix``)."""
+    GPU (``ed_scan_at_irrep_matrix``).
+
+    Checkpoints (``:1137-1196``): a plain scan with ``checkpoint_path`` loads and extends that
+    file; a flux scan with ``checkpoint_dir`` writes one file per twist, named by
+    ``ed_scan_checkpoint_filename`` (the model's parameter digest), and a twist whose file
+    exists is resumed (after checking it holds the same Hamiltonian) instead of restarted.
+    ``overwrite`` discards stored results.  Each returned ``EDData`` names its file in
+    ``checkpoint_path``."""
     if twisted_phases_over_2pi_list is None:
-        return _scan_sectors(ed_data, nev, mode, scanned_sectors, **solve_kw)
-    from .lattice import identity_group
+        return _scan_sectors(ed_data, nev, mode, scanned_sectors, checkpoint_path=checkpoint_path,
+                             overwrite=overwrite, solve=solve, **solve_kw)
     model = ed_data.model
     if not 1 <= flux_direction <= 2:
         raise ValueError("flux_direction must be 1 or 2")
     out = []
-    ctx = ctx or ed_data.orbit_catalog.ctx
+    if ctx is None and solve is None:
+        ctx = ed_data.orbit_catalog.ctx
+    identity = ed_data.symmetry_group.name == "identity"
     for t in twisted_phases_over_2pi_list:
         flux = [0.0, 0.0]
         flux[flux_direction - 1] = float(t)
         m = twisted(model, flux)
         m.n_filled = ed_data.n_filled
-        G = identity_group(m.n_site) if ed_data.symmetry_group.name == "identity" else \
-            model_group(m, flux)
-        d = build_ed_data(m, symmetry_group=G, ctx=ctx)
-        out.append(_scan_sectors(d, nev, mode, scanned_sectors, **solve_kw))
+        path = None
+        if checkpoint_dir is not None:
+            L1, L2 = m.lattice.sample_size
+            path = os.path.join(checkpoint_dir, _ckpt.ed_scan_checkpoint_filename(
+                m, flux, Fraction(ed_data.n_filled, L1 * L2), identity))
+            if os.path.isfile(path) and not overwrite:
+                cached = _ckpt.load_checkpoint(path)
+                if not _ckpt.same_flux_problem(cached.model, m):
+                    raise RuntimeError("Flux checkpoint belongs to a different Hamiltonian.")
+                out.append(resume_ed_scan(path, nev, mode, scanned_sectors, ctx=ctx, solve=solve, **solve_kw))
+                continue
+        G = identity_group(m.n_site) if identity else model_group(m, flux)
+        d = build_ed_data(m, symmetry_group=G, ctx=ctx, build_catalog=solve is None)
+        out.append(_scan_sectors(d, nev, mode, scanned_sectors, checkpoint_path=path, overwrite=overwrite,
+                                 solve=solve, **solve_kw))
     return out
 
 

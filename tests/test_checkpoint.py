@@ -1,0 +1,198 @@
+"""Checkpointed sector scans on the CPU (save_checkpoint / load_checkpoint /
+ed_scan_checkpoint_filename / _same_flux_problem / resume_ed_scan!,
+src/symmetry_resolved_ed.jl:1137-1277), with the oracle's dense sector solve standing in for
+the GPU so that the driver logic runs here: a scan stopped midway resumes without
+recomputing finished sectors and ends with the results of an uninterrupted scan; a failed
+write leaves the previous checkpoint whole and no temporary file; files of another solver
+revision or of another Hamiltonian are refused; a flux scan writes one file per twist under
+its parameter-digest name and a rerun resumes each twist."""
+import os
+import re
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from realspace_exactdiagonalization_amd import checkpoint as CK
+from realspace_exactdiagonalization_amd import ed as E
+from realspace_exactdiagonalization_amd import lattice as L
+
+
+class Stop(Exception):
+    pass
+
+
+class OracleSolver:
+    """solve(ed_data, irrep_index, nev) -> (vals, vecs, dim) from the oracle's dense sector;
+    raises Stop after `stop_after` solves when set (a job killed mid-scan)."""
+
+    def __init__(self, oracle_mod, stop_after=None):
+        self.O = oracle_mod
+        self.calls = []
+        self.stop_after = stop_after
+        self._cats = {}
+
+    def __call__(self, d, idx, nev):
+        if self.stop_after is not None and len(self.calls) >= self.stop_after:
+            raise Stop()
+        m = d.model
+        key = id(d)
+        if key not in self._cats:
+            self._cats[key] = self.O.Catalog(m.n_site, d.n_filled, d.symmetry_group, m.statistics == L.FERMIONIC)
+        sec = self.O.Sector(self._cats[key], d.irrep_list[idx].values)
+        self.calls.append((tuple(m.theta), tuple(d.irrep_list[idx].label)))
+        if sec.dim == 0:
+            return np.zeros(0), np.zeros((0, 0), dtype=np.complex128), 0
+        sec.set_terms(*L.term_arrays(m))
+        H = sec.dense()
+        w, U = np.linalg.eigh(0.5 * (H + H.conj().T))
+        k = min(nev, sec.dim)
+        return w[:k], U[:, :k], sec.dim
+
+
+def model(theta=(0.23, 0.17), V1=1.0):
+    return L.twisted(L.checkerboard(3, 2, n_filled=4, V1=V1, V2=0.5), theta)
+
+
+def ed_data(m):
+    return E.build_ed_data(m, symmetry_group=L.model_group(m, m.theta), build_catalog=False)
+
+
+def test_fnv1a64_published_vectors():
+    assert CK.fnv1a64(b"") == 0xCBF29CE484222325
+    assert CK.fnv1a64(b"a") == 0xAF63DC4C8601EC8C
+    assert CK.fnv1a64(b"foobar") == 0x85944171F73967E8
+
+
+def test_julia_repr_of_parameter_pairs():
+    assert CK.julia_float(1.0) == "1.0" and CK.julia_float(-0.58) == "-0.58"
+    assert CK.julia_float(1e-5) == "1.0e-5" and CK.julia_float(0.0001) == "0.0001"
+    assert CK.julia_float(1e6) == "1.0e6" and CK.julia_float(123456.0) == "123456.0"
+    assert CK.julia_repr(0.4 * np.pi) == "1.2566370614359172"
+    p = {"t": 1.0, "phi_over_2pi": 0.125, "V1": 1.0}
+    assert CK._pairs_repr(p) == '["V1" => 1.0, "phi_over_2pi" => 0.125, "t" => 1.0]'
+    assert CK._pairs_repr({"a": 1, "b": 0.5}) == 'Pair{String, Real}["a" => 1, "b" => 0.5]'
+
+
+def test_checkpoint_filename_format():
+    m = model((0.25, 0.0))
+    name = CK.ed_scan_checkpoint_filename(m, [0.25, 0.0], Fraction(4, 6))
+    h = CK.fnv1a64(CK._pairs_repr(m.params).encode())
+    assert name == f"solver_v2_checkerboard_[3, 2]_nu=2_3_flux=[0.25, 0.0]_p={h:x}.npz"
+    assert CK.ed_scan_checkpoint_filename(m, [0.25, 0.0], Fraction(4, 6), identity=True) == "identity_" + name
+    # another parameter value gives another digest
+    m2 = model((0.25, 0.0), V1=0.9)
+    assert CK.ed_scan_checkpoint_filename(m2, [0.25, 0.0], Fraction(4, 6)) != name
+
+
+def test_scan_stopped_midway_resumes_without_recomputing(oracle_mod, tmp_path):
+    m = model()
+    path = str(tmp_path / "scan.npz")
+    full = E.ed_scan(ed_data(m), nev=3, mode="matrixfree", solve=OracleSolver(oracle_mod))
+    n_sec = len(full.irrep_list)
+    first = OracleSolver(oracle_mod, stop_after=3)
+    with pytest.raises(Stop):
+        E.ed_scan(ed_data(m), nev=3, mode="matrixfree", checkpoint_path=path, solve=first)
+    st = CK.load_checkpoint(path)
+    assert sorted(st.ed_scan_res) == [0, 1, 2] and not os.path.exists(path + ".tmp")
+    second = OracleSolver(oracle_mod)
+    res = E.resume_ed_scan(path, nev=3, mode="matrixfree", solve=second)
+    done_first = {c[1] for c in first.calls}
+    assert len(second.calls) == n_sec - 3 and not done_first & {c[1] for c in second.calls}
+    assert res.checkpoint_path == path and res.sector_dims == full.sector_dims
+    for i in range(n_sec):
+        assert np.allclose(res.ed_scan_res[i][0], full.ed_scan_res[i][0], rtol=0, atol=1e-12)
+    # everything finished: a further resume solves nothing; asking for more pairs redoes all
+    again = OracleSolver(oracle_mod)
+    E.resume_ed_scan(path, nev=3, mode="matrixfree", solve=again)
+    assert again.calls == []
+    more = OracleSolver(oracle_mod)
+    E.resume_ed_scan(path, nev=4, mode="matrixfree", solve=more)
+    assert len(more.calls) == sum(1 for d in full.sector_dims if d > 3)
+
+
+def test_failed_write_keeps_the_previous_checkpoint(oracle_mod, tmp_path, monkeypatch):
+    m = model()
+    path = str(tmp_path / "scan.npz")
+    with pytest.raises(Stop):
+        E.ed_scan(ed_data(m), nev=2, mode="matrixfree", checkpoint_path=path, solve=OracleSolver(oracle_mod, 2))
+    before = open(path, "rb").read()
+
+    def broken(*a, **k):
+        raise OSError("disk full")
+    monkeypatch.setattr(np, "savez", broken)
+    with pytest.raises(OSError):
+        E.resume_ed_scan(path, nev=2, mode="matrixfree", solve=OracleSolver(oracle_mod))
+    assert open(path, "rb").read() == before and not os.path.exists(path + ".tmp")
+    assert sorted(CK.load_checkpoint(path).ed_scan_res) == [0, 1]
+
+
+def test_legacy_and_incompatible_checkpoints_are_refused(oracle_mod, tmp_path):
+    m = model()
+    path = str(tmp_path / "scan.npz")
+    with pytest.raises(Stop):
+        E.ed_scan(ed_data(m), nev=2, mode="matrixfree", checkpoint_path=path, solve=OracleSolver(oracle_mod, 2))
+    with pytest.raises(RuntimeError, match="incompatible"):
+        E.ed_scan(ed_data(model(V1=0.7)), nev=2, mode="matrixfree", checkpoint_path=path,
+                  solve=OracleSolver(oracle_mod))
+    E.ed_scan(ed_data(model(V1=0.7)), nev=2, mode="matrixfree", checkpoint_path=path, overwrite=True,
+              solve=OracleSolver(oracle_mod))
+    assert CK.same_flux_problem(CK.load_checkpoint(path).model, model(V1=0.7))
+    # a file of another solver revision
+    st = CK.load_checkpoint(path)
+    old = CK.SOLVER_REVISION
+    try:
+        CK.SOLVER_REVISION = 1
+        CK.save_checkpoint(st, path)
+    finally:
+        CK.SOLVER_REVISION = old
+    with pytest.raises(RuntimeError, match="Legacy"):
+        CK.load_checkpoint(path)
+
+
+def test_same_flux_problem():
+    a, b = model(), model()
+    assert CK.same_flux_problem(a, b)
+    b.bilinear_terms = list(reversed(b.bilinear_terms))
+    assert CK.same_flux_problem(a, b)
+    c = model()
+    i, j, t = c.bilinear_terms[0]
+    c.bilinear_terms[0] = (i, j, t * (1 + 1e-9))
+    assert not CK.same_flux_problem(a, c)
+    assert not CK.same_flux_problem(a, model(theta=(0.23, 0.18)))
+    rt = CK.model_from_json(CK.model_to_json(a))
+    assert CK.same_flux_problem(a, rt) and rt.n_filled == a.n_filled
+
+
+def test_flux_scan_writes_one_file_per_twist_and_resumes(oracle_mod, tmp_path):
+    base = L.checkerboard(3, 2, n_filled=4, V1=1.0, V2=0.5)
+    d0 = ed_data(base)
+    thetas = [0.0, 0.25, 0.5]
+    ck = str(tmp_path / "ck")
+    ref = E.ed_scan(ed_data(base), nev=2, mode="matrixfree", twisted_phases_over_2pi_list=thetas,
+                    solve=OracleSolver(oracle_mod))
+    n_sec = len(d0.irrep_list)
+    s1 = OracleSolver(oracle_mod, stop_after=n_sec + 2)   # dies two sectors into theta = 0.25
+    with pytest.raises(Stop):
+        E.ed_scan(ed_data(base), nev=2, mode="matrixfree", twisted_phases_over_2pi_list=thetas,
+                  checkpoint_dir=ck, solve=s1)
+    files = sorted(os.listdir(ck))
+    assert len(files) == 2 and all(re.match(r"solver_v2_checkerboard_\[3, 2\]_nu=2_3_flux=\[0\.(0|25), 0\.0\]_p=[0-9a-f]+\.npz$", f)
+                                   for f in files)
+    s2 = OracleSolver(oracle_mod)
+    out = E.ed_scan(ed_data(base), nev=2, mode="matrixfree", twisted_phases_over_2pi_list=thetas,
+                    checkpoint_dir=ck, solve=s2)
+    assert len(s2.calls) == 2 * n_sec - 2 and len(os.listdir(ck)) == 3
+    assert not [c for c in s2.calls if c[0] == (0.0, 0.0)]
+    for r, o in zip(ref, out):
+        for i in range(n_sec):
+            assert np.allclose(r.ed_scan_res[i][0], o.ed_scan_res[i][0], rtol=0, atol=1e-12)
+    assert out[1].checkpoint_path == os.path.join(ck, CK.ed_scan_checkpoint_filename(
+        L.twisted(base, (0.25, 0.0)), [0.25, 0.0], Fraction(4, 6)))
+    # a file under that name holding another Hamiltonian is refused
+    st = CK.load_checkpoint(out[2].checkpoint_path)
+    st.model.density_terms = [(i, j, 2 * v) for i, j, v in st.model.density_terms]
+    CK.save_checkpoint(st, out[2].checkpoint_path)
+    with pytest.raises(RuntimeError, match="different Hamiltonian"):
+        E.ed_scan(ed_data(base), nev=2, mode="matrixfree", twisted_phases_over_2pi_list=thetas,
+                  checkpoint_dir=ck, solve=OracleSolver(oracle_mod))

@@ -7,6 +7,7 @@ stabilizer orders); rep amplitudes exact at zero twist and within 1e-13 otherwis
 (FP64) and 2e-6 relative (FP32); eigenvalues within 1e-10 relative.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -951,3 +952,28 @@ def test_ed_scan_sectors_filter_resume_and_flux_mode(ed, ctx, oracle_mod):
             a = d.ed_scan_res[idx][0]
             r = ref[((t, 0.0), tuple(ir.label))]
             assert np.abs(a - r[:len(a)]).max() <= 1e-10 * np.abs(r).max()
+
+
+def test_checkpointed_gpu_scan_resumes(ed, ctx, tmp_path):
+    """A GPU flux scan with per-twist checkpoint files (ed_scan! flux mode with
+    checkpoints, src/symmetry_resolved_ed.jl:1320-1355): a first run limited to two momentum
+    sectors, as a job stopped partway would leave it, then the full run over the same
+    directory resumes each twist (resume_ed_scan!, :1266-1277) and ends with the
+    eigenvalues of an uninterrupted scan; the stored eigenvectors are the GPU's."""
+    base = L.checkerboard(3, 2, n_filled=4, V1=1.0, V2=0.5)
+    thetas = [0.0, 0.3]
+    ck = str(tmp_path / "ck")
+    full = ed.ed_scan(ed.build_ed_data(base, ctx=ctx), nev=2, mode="matrixfree",
+                      twisted_phases_over_2pi_list=thetas, ctx=ctx)
+    part = ed.ed_scan(ed.build_ed_data(base, ctx=ctx), nev=2, mode="matrixfree", scanned_sectors=[(0, 0), (1, 1)],
+                      twisted_phases_over_2pi_list=thetas, ctx=ctx, checkpoint_dir=ck)
+    assert all(sorted(d.ed_scan_res) == [0, 3] for d in part)
+    out = ed.ed_scan(ed.build_ed_data(base, ctx=ctx), nev=2, mode="matrixfree",
+                     twisted_phases_over_2pi_list=thetas, ctx=ctx, checkpoint_dir=ck)
+    for d, f, p in zip(out, full, part):
+        assert d.checkpoint_path == p.checkpoint_path and os.path.isfile(d.checkpoint_path)
+        for i in range(len(d.irrep_list)):
+            a, b = d.ed_scan_res[i][0], f.ed_scan_res[i][0]
+            assert np.abs(a - b).max() <= 1e-10 * max(1.0, np.abs(b).max())
+        for i in (0, 3):   # carried over from the partial run, not recomputed
+            assert np.array_equal(d.ed_scan_res[i][1], p.ed_scan_res[i][1])
