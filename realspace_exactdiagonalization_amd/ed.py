@@ -391,9 +391,8 @@ class EDData:
 
     @property
     def filling_fraction(self) -> Fraction:
-        """Particles per unit cell (the reference's filling_fraction)."""
-        L1, L2 = self.model.lattice.sample_size
-        return Fraction(self.n_filled, L1 * L2)
+        """Particles per vertex, n_filled / n_site (the reference's filling_fraction)."""
+        return Fraction(self.n_filled, self.model.n_site)
 
 
 def build_ed_data(model: Model, symmetry_group: Optional[SymmetryGroup] = None,
@@ -538,9 +537,8 @@ def ed_scan(ed_data: EDData, nev: int = 5, mode: str = "matrix", scanned_sectors
         m.n_filled = ed_data.n_filled
         path = None
         if checkpoint_dir is not None:
-            L1, L2 = m.lattice.sample_size
             path = os.path.join(checkpoint_dir, _ckpt.ed_scan_checkpoint_filename(
-                m, flux, Fraction(ed_data.n_filled, L1 * L2), identity))
+                m, flux, Fraction(ed_data.n_filled, m.n_site), identity))
             if os.path.isfile(path) and not overwrite:
                 cached = _ckpt.load_checkpoint(path)
                 if not _ckpt.same_flux_problem(cached.model, m):

@@ -76,13 +76,13 @@ def test_julia_repr_of_parameter_pairs():
 
 def test_checkpoint_filename_format():
     m = model((0.25, 0.0))
-    name = CK.ed_scan_checkpoint_filename(m, [0.25, 0.0], Fraction(4, 6))
+    name = CK.ed_scan_checkpoint_filename(m, [0.25, 0.0], Fraction(4, 12))
     h = CK.fnv1a64(CK._pairs_repr(m.params).encode())
-    assert name == f"solver_v2_checkerboard_[3, 2]_nu=2_3_flux=[0.25, 0.0]_p={h:x}.npz"
-    assert CK.ed_scan_checkpoint_filename(m, [0.25, 0.0], Fraction(4, 6), identity=True) == "identity_" + name
+    assert name == f"solver_v2_checkerboard_[3, 2]_nu=1_3_flux=[0.25, 0.0]_p={h:x}.npz"
+    assert CK.ed_scan_checkpoint_filename(m, [0.25, 0.0], Fraction(4, 12), identity=True) == "identity_" + name
     # another parameter value gives another digest
     m2 = model((0.25, 0.0), V1=0.9)
-    assert CK.ed_scan_checkpoint_filename(m2, [0.25, 0.0], Fraction(4, 6)) != name
+    assert CK.ed_scan_checkpoint_filename(m2, [0.25, 0.0], Fraction(4, 12)) != name
 
 
 def test_scan_stopped_midway_resumes_without_recomputing(oracle_mod, tmp_path):
@@ -177,7 +177,7 @@ def test_flux_scan_writes_one_file_per_twist_and_resumes(oracle_mod, tmp_path):
         E.ed_scan(ed_data(base), nev=2, mode="matrixfree", twisted_phases_over_2pi_list=thetas,
                   checkpoint_dir=ck, solve=s1)
     files = sorted(os.listdir(ck))
-    assert len(files) == 2 and all(re.match(r"solver_v2_checkerboard_\[3, 2\]_nu=2_3_flux=\[0\.(0|25), 0\.0\]_p=[0-9a-f]+\.npz$", f)
+    assert len(files) == 2 and all(re.match(r"solver_v2_checkerboard_\[3, 2\]_nu=1_3_flux=\[0\.(0|25), 0\.0\]_p=[0-9a-f]+\.npz$", f)
                                    for f in files)
     s2 = OracleSolver(oracle_mod)
     out = E.ed_scan(ed_data(base), nev=2, mode="matrixfree", twisted_phases_over_2pi_list=thetas,
@@ -188,7 +188,7 @@ def test_flux_scan_writes_one_file_per_twist_and_resumes(oracle_mod, tmp_path):
         for i in range(n_sec):
             assert np.allclose(r.ed_scan_res[i][0], o.ed_scan_res[i][0], rtol=0, atol=1e-12)
     assert out[1].checkpoint_path == os.path.join(ck, CK.ed_scan_checkpoint_filename(
-        L.twisted(base, (0.25, 0.0)), [0.25, 0.0], Fraction(4, 6)))
+        L.twisted(base, (0.25, 0.0)), [0.25, 0.0], Fraction(4, 12)))
     # a file under that name holding another Hamiltonian is refused
     st = CK.load_checkpoint(out[2].checkpoint_path)
     st.model.density_terms = [(i, j, 2 * v) for i, j, v in st.model.density_terms]
