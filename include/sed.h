@@ -80,7 +80,8 @@ int sed_destroy(sed_context* ctx);
  * _inverse_operation_indices (:109-117).  The site phases must be of gauge form
  * eta_g(i) = f(i)/f(perm_g(i)) (true of build_translation_group at any twist,
  * :1374-1405); otherwise SED_EUNSUPPORTED.  Requires n_site <= 64,
- * C(n_site, n_filled) < 2^31 and n_group <= 64. */
+ * C(n_site, n_filled) <= 2^32 - 2 (colex ranks are 32-bit), at most 2^31 orbits and
+ * n_group <= 64.  Sectors built on it hold at most 2^31 - 1 rows. */
 int sed_catalog_build(sed_context* ctx, int n_site, int n_filled, int statistics,
                       int n_group, int identity_idx, const int32_t* perm, const double* phases,
                       sed_catalog** out);

@@ -232,8 +232,9 @@ int sed_catalog_build(sed_context* ctx, int n_site, int n_filled, int statistics
             }
         }
         tot = C[n_site][n_filled];
-        if (tot >= (1ull << 31))
-            return fail(SED_EUNSUPPORTED, "C(n_site, n_filled) = " + std::to_string(tot) + " exceeds 2^31");
+        // ranks are uint32 (the binomial table saturates at 2^32 - 1, above every rank)
+        if (tot >= 0xffffffffull)
+            return fail(SED_EUNSUPPORTED, "C(n_site, n_filled) = " + std::to_string(tot) + " exceeds 2^32 - 2");
         cat->bstride = n_filled + 2;
         cat->binom.assign((size_t)(n_site + 1) * cat->bstride, 0);
         for (int a = 0; a <= n_site; ++a)
@@ -824,6 +825,8 @@ int sed_sector_one_body(sed_sector* sec, int which, double* rho) {
     tmp.d_rep.borrow(sec->d_rep);
     tmp.d_stab.borrow(sec->d_stab);
     tmp.d_table.borrow(sec->d_table);
+    tmp.wide = sec->wide;
+    tmp.d_table64.borrow(sec->d_table64);
     tmp.d_chi64.borrow(sec->d_chi64);
     tmp.d_chi32.borrow(sec->d_chi32);
     tmp.d_chi2_64.borrow(sec->d_chi2_64);
@@ -884,6 +887,8 @@ int sed_sector_matrix_csc(sed_sector* sec, int64_t* nnz, int64_t* colptr, int64_
     if (!sec->has_terms) return fail(SED_ESTATE, "matrix before sed_sector_set_terms");
     const bool query = !colptr && !rowval && !nzval;
     if (!query && (!colptr || !rowval || !nzval)) return fail(SED_EINVAL, "colptr, rowval and nzval: all or none");
+    if (sec->wide) return fail(SED_EUNSUPPORTED, "sparse matrix of a sector with an 8-byte table (dimension " +
+                                                     std::to_string(sec->dim) + ")");
     SED_HIP(hipSetDevice(sec->cat->ctx->device));
     if (!sec->csc_ready) {
         SED_TRY(build_sector_csc(sec, sec->csc_colptr, sec->csc_rowval, sec->csc_nzval));
@@ -962,6 +967,7 @@ int sed_sector_position_operator(sed_sector* to, sed_sector* from, const double*
                                  double* vals) {
     if (!to || !from || !site_phases || !rows || !vals) return fail(SED_EINVAL, "NULL argument");
     if (to->cat != from->cat) return fail(SED_EINVAL, "sectors of different catalogs (symmetry groups)");
+    if (to->wide) return fail(SED_EUNSUPPORTED, "position operator into a sector with an 8-byte table");
     const sed_catalog* cat = from->cat;
     SED_HIP(hipSetDevice(cat->ctx->device));
     hipStream_t s = cat->ctx->stream;

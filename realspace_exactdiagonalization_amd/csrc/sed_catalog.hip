@@ -18,6 +18,7 @@
 // per-rank pass writing the packed 4-byte sector table consumed by H|psi>.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "sed_device.hpp"
 #include "sed_internal.hpp"
@@ -189,7 +190,10 @@ static size_t cat_lds_bytes(const sed_catalog* cat) {
 // ------------------------------------------------------------------------
 // 1. per-rank representative, group element and sign
 // ------------------------------------------------------------------------
-template <typename M, bool FERMI>
+// Catalogs of 2^31 <= C(N,k) < 2^32 ranks (BIG) cannot keep a representative's rank in
+// the int32 rank table: this pass then stores only whether rank r is its own
+// representative (1) or not (0), and k_cat_final_big fills in the orbits after the scan.
+template <typename M, bool FERMI, bool BIG>
 __global__ void __launch_bounds__(kCatBlock) k_cat_images(CatConst c, int64_t n_total,
                                                           int32_t* __restrict__ rep_rank,
                                                           uint8_t* __restrict__ sym_par) {
@@ -212,7 +216,8 @@ __global__ void __launch_bounds__(kCatBlock) k_cat_images(CatConst c, int64_t n_
                 best_par = par;
             }
         }
-        rep_rank[r] = (int32_t)colex_rank<M>(best, bin, c.bs);
+        if (BIG) rep_rank[r] = best == s ? 1 : 0;
+        else rep_rank[r] = (int32_t)colex_rank<M>(best, bin, c.bs);
         sym_par[r] = (uint8_t)(best_h | (best_par << 7));
     }
 }
@@ -221,13 +226,17 @@ struct RepFlag {
     const int32_t* rr;
     __device__ uint32_t operator()(int64_t i) const { return rr[i] == (int32_t)i ? 1u : 0u; }
 };
+struct RepFlagBig {
+    const int32_t* rr;
+    __device__ uint32_t operator()(int64_t i) const { return rr[i] == 1 ? 1u : 0u; }
+};
 struct RepWriter {
     int32_t* rr;
-    int32_t* orbit_rank;
+    int64_t* orbit_rank;
     __device__ void operator()(int64_t i, uint32_t fl, uint32_t ex) const {
         if (fl) {
             rr[i] = -(int32_t)ex - 1;
-            orbit_rank[ex] = (int32_t)i;
+            orbit_rank[ex] = i;
         }
     }
 };
@@ -241,10 +250,32 @@ __global__ void __launch_bounds__(256) k_cat_final(int64_t n_total, int32_t* __r
     }
 }
 
+// 3 (BIG). every rank that is not a representative takes its representative's orbit
+// number: the representative (minimal image) is found again and its rank, where the scan
+// wrote -(orbit+1), read.  Representatives are not rewritten.
+template <typename M, bool FERMI>
+__global__ void __launch_bounds__(kCatBlock) k_cat_final_big(CatConst c, int64_t n_total, int32_t* __restrict__ rr) {
+    uint32_t* bin;
+    uint8_t *perm, *inv;
+    stage_cat(c, bin, perm, inv);
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_total;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        if (rr[r] < 0) continue;
+        const M s = colex_unrank<M>((uint32_t)r, c.n, c.k, bin, c.bs);
+        M best = ~(M)0;
+        for (int h = 0; h < c.nG; ++h) {
+            int par;
+            const M img = apply_perm<M, FERMI>(s, perm + h * c.n, par);
+            best = img < best ? img : best;
+        }
+        rr[r] = rr[colex_rank<M>(best, bin, c.bs)];
+    }
+}
+
 // 4. per-orbit representative mask, stabilizer set and stabilizer signs
 template <typename M, bool FERMI>
 __global__ void __launch_bounds__(kCatBlock) k_cat_orbits(CatConst c, int64_t n_orbits,
-                                                          const int32_t* __restrict__ orbit_rank,
+                                                          const int64_t* __restrict__ orbit_rank,
                                                           uint64_t* __restrict__ orep,
                                                           uint64_t* __restrict__ ostab,
                                                           uint64_t* __restrict__ ostab_par) {
@@ -305,7 +336,7 @@ __global__ void __launch_bounds__(kCatBlock) k_cat_amp(CatConst c, const double2
     }
 }
 
-template <typename M, bool F>
+template <typename M, bool F, bool BIG>
 static int run_catalog(sed_catalog* cat) {
     hipStream_t s = cat->ctx->stream;
     CatConst c{cat->d_binom.p, cat->d_perm.p, cat->d_inv.p, cat->n_site, cat->n_filled, cat->n_group,
@@ -315,18 +346,26 @@ static int run_catalog(sed_catalog* cat) {
     SED_TRY(cat->d_rep_rank.alloc(n));
     SED_TRY(cat->d_sym_par.alloc(n));
     int64_t nblk = std::min<int64_t>((n + kCatBlock - 1) / kCatBlock, 1 << 22);
-    hipLaunchKernelGGL((k_cat_images<M, F>), dim3((unsigned)nblk), dim3(kCatBlock), lds, s, c, n,
+    hipLaunchKernelGGL((k_cat_images<M, F, BIG>), dim3((unsigned)nblk), dim3(kCatBlock), lds, s, c, n,
                        cat->d_rep_rank.p, cat->d_sym_par.p);
     SED_HIP(hipGetLastError());
     ScanPlan plan;
-    RepFlag flag{cat->d_rep_rank.p};
+    using Flag = typename std::conditional<BIG, RepFlagBig, RepFlag>::type;
+    Flag flag{cat->d_rep_rank.p};
     SED_TRY(scan_prepare(flag, n, s, plan));
     cat->n_orbits = plan.total;
-    DevBuf<int32_t> orbit_rank;
+    if (cat->n_orbits > ((int64_t)1 << 31))   // rank entries hold -(orbit + 1) in int32
+        return fail(SED_EUNSUPPORTED, std::to_string(cat->n_orbits) + " orbits exceed 2^31");
+    DevBuf<int64_t> orbit_rank;
     SED_TRY(orbit_rank.alloc(cat->n_orbits));
     SED_TRY(scan_write(flag, RepWriter{cat->d_rep_rank.p, orbit_rank.p}, n, s, plan));
-    nblk = std::min<int64_t>((n + 255) / 256, 1 << 22);
-    hipLaunchKernelGGL(k_cat_final, dim3((unsigned)nblk), dim3(256), 0, s, n, cat->d_rep_rank.p);
+    if (BIG) {
+        hipLaunchKernelGGL((k_cat_final_big<M, F>), dim3((unsigned)nblk), dim3(kCatBlock), lds, s, c, n,
+                           cat->d_rep_rank.p);
+    } else {
+        nblk = std::min<int64_t>((n + 255) / 256, 1 << 22);
+        hipLaunchKernelGGL(k_cat_final, dim3((unsigned)nblk), dim3(256), 0, s, n, cat->d_rep_rank.p);
+    }
     SED_HIP(hipGetLastError());
     SED_TRY(cat->d_orbit_rep.alloc(cat->n_orbits));
     SED_TRY(cat->d_orbit_stab.alloc(cat->n_orbits));
@@ -342,8 +381,11 @@ static int run_catalog(sed_catalog* cat) {
 
 int launch_catalog(sed_catalog* cat) {
     const bool f = cat->stats == SED_FERMIONIC;
-    if (cat->n_site <= 32) return f ? run_catalog<uint32_t, true>(cat) : run_catalog<uint32_t, false>(cat);
-    return f ? run_catalog<uint64_t, true>(cat) : run_catalog<uint64_t, false>(cat);
+    // C(32, k) < 2^31: only catalogs of more than 32 sites can be BIG
+    if (cat->n_site <= 32) return f ? run_catalog<uint32_t, true, false>(cat) : run_catalog<uint32_t, false, false>(cat);
+    if (cat->n_total >= ((int64_t)1 << 31))
+        return f ? run_catalog<uint64_t, true, true>(cat) : run_catalog<uint64_t, false, true>(cat);
+    return f ? run_catalog<uint64_t, true, false>(cat) : run_catalog<uint64_t, false, false>(cat);
 }
 
 template <typename M, bool F>
@@ -418,16 +460,17 @@ struct SectorWriter {
     }
 };
 
+template <typename TE>
 __global__ void __launch_bounds__(256) k_sec_table(int64_t n_total, int shift, const int32_t* __restrict__ rr,
                                                    const uint8_t* __restrict__ sym_par,
                                                    const int32_t* __restrict__ row_of,
-                                                   uint32_t* __restrict__ table) {
+                                                   TE* __restrict__ table) {
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_total;
          r += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t o = -rr[r] - 1;
+        const int64_t o = -(int64_t)rr[r] - 1;
         const int32_t row = row_of[o];
         const uint32_t sp = sym_par[r];
-        table[r] = row < 0 ? kNoRow : (((uint32_t)row << shift) | ((sp & 0x7fu) << 1) | (sp >> 7));
+        table[r] = row < 0 ? ~(TE)0 : (((TE)row << shift) | (TE)(((sp & 0x7fu) << 1) | (sp >> 7)));
     }
 }
 
@@ -449,10 +492,13 @@ int launch_sector(sed_sector* sec, const double* chi_dev) {
     int gbits = 0;
     while ((1 << gbits) < cat->n_group) ++gbits;
     sec->shift = gbits + 1;
-    if (sec->dim > 0 && (uint64_t)sec->dim > ((1ull << (32 - sec->shift)) - 1))
-        return fail(SED_EUNSUPPORTED, "sector dimension " + std::to_string(sec->dim) +
-                                          " does not fit the 4-byte table entry with |G| = " +
-                                          std::to_string(cat->n_group));
+    // rows are int32 (2^31 complex doubles are 32 GiB per Krylov vector); a row that does
+    // not fit the 4-byte entry beside the group bits takes the 8-byte table
+    if (sec->dim >= ((int64_t)1 << 31))
+        return fail(SED_EUNSUPPORTED, "sector dimension " + std::to_string(sec->dim) + " exceeds 2^31 - 1");
+    sec->wide = sec->dim > 0 && (uint64_t)sec->dim > ((1ull << (32 - sec->shift)) - 1);
+    if (sec->wide && cat->n_site <= 32)   // C(32, k) / |G| always fits the 4-byte entry
+        return fail(SED_EUNSUPPORTED, "8-byte sector table on a catalog of at most 32 sites");
     DevBuf<int32_t> row_of;
     SED_TRY(row_of.alloc(no));
     SED_TRY(sec->d_rep.alloc(sec->dim));
@@ -460,10 +506,16 @@ int launch_sector(sed_sector* sec, const double* chi_dev) {
     SED_TRY(scan_write(fl, SectorWriter{row_of.p, cat->d_orbit_rep.p, cat->d_orbit_stab.p, sec->d_rep.p,
                                         sec->d_stab.p},
                        no, s, plan));
-    SED_TRY(sec->d_table.alloc(cat->n_total));
     nblk = std::min<int64_t>((cat->n_total + 255) / 256, 1 << 22);
-    hipLaunchKernelGGL(k_sec_table, dim3((unsigned)nblk), dim3(256), 0, s, cat->n_total, sec->shift,
-                       cat->d_rep_rank.p, cat->d_sym_par.p, row_of.p, sec->d_table.p);
+    if (sec->wide) {
+        SED_TRY(sec->d_table64.alloc(cat->n_total));
+        hipLaunchKernelGGL(k_sec_table<uint64_t>, dim3((unsigned)nblk), dim3(256), 0, s, cat->n_total, sec->shift,
+                           cat->d_rep_rank.p, cat->d_sym_par.p, row_of.p, sec->d_table64.p);
+    } else {
+        SED_TRY(sec->d_table.alloc(cat->n_total));
+        hipLaunchKernelGGL(k_sec_table<uint32_t>, dim3((unsigned)nblk), dim3(256), 0, s, cat->n_total, sec->shift,
+                           cat->d_rep_rank.p, cat->d_sym_par.p, row_of.p, sec->d_table.p);
+    }
     SED_HIP(hipGetLastError());
     SED_HIP(hipStreamSynchronize(s));
     return SED_OK;

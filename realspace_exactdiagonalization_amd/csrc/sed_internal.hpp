@@ -137,6 +137,10 @@ struct sed_sector {
     sed::DevBuf<uint64_t> d_rep;      // sector representatives (ascending)
     sed::DevBuf<uint8_t> d_stab;      // stabilizer orders
     sed::DevBuf<uint32_t> d_table;    // per rank: packed (row, g, parity) or 0xffffffff
+    // sectors whose row << shift does not fit 32 bits (more than 32 sites only) keep the
+    // same entry in 8 bytes: d_table64 (no row = all ones) and wide = true, d_table empty
+    bool wide = false;
+    sed::DevBuf<uint64_t> d_table64;
     // terms
     bool has_terms = false;
     int n_terms = 0, n_dens = 0;

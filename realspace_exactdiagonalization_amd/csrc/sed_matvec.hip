@@ -44,10 +44,10 @@ namespace sed {
 // fixed xor butterfly over the row's lanes.  Hops are processed BATCH at a time: BATCH
 // independent table reads are issued before the BATCH dependent gathers, so each lane
 // keeps several memory requests in flight.
-template <typename M, typename R, bool FERMI, int LPR, int BATCH, int BLOCK, bool FLAT>
+template <typename M, typename R, bool FERMI, int LPR, int BATCH, int BLOCK, bool FLAT, typename TE>
 __global__ void __launch_bounds__(BLOCK) k_matvec(MvConst c, const uint64_t* __restrict__ rep,
                                                   const uint8_t* __restrict__ stab,
-                                                  const uint32_t* __restrict__ table,
+                                                  const TE* __restrict__ table,
                                                   const typename C2<R>::T* __restrict__ xhat,
                                                   typename C2<R>::T* __restrict__ y, int64_t nblocks,
                                                   int64_t chunk) {
@@ -62,10 +62,11 @@ __global__ void __launch_bounds__(BLOCK) k_matvec(MvConst c, const uint64_t* __r
     const M m = live ? (M)rep[r] : (M)0;
     R ax = 0, ay = 0, dx = 0, dy = 0;
     if constexpr (FLAT)
-        row_sums_flat<M, R, FERMI, LPR, BATCH, BLOCK>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx, dy);
+        row_sums_flat<M, R, FERMI, LPR, BATCH, BLOCK, TE>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx,
+                                                          dy);
     else
-        row_sums_padded<M, R, FERMI, LPR, BATCH, BLOCK>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx,
-                                                        dy);
+        row_sums_padded<M, R, FERMI, LPR, BATCH, BLOCK, TE>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay,
+                                                            dx, dy);
     if (LPR > 1) {
 #pragma unroll
         for (int off = LPR / 2; off >= 1; off >>= 1) {
@@ -154,9 +155,9 @@ __global__ void __launch_bounds__(kMvBlock) k_dense(MvConst c, const uint64_t* _
 }
 
 // Exact counts of gated hops (one table read each) and in-sector hops (one gather).
-template <typename M>
+template <typename M, typename TE>
 __global__ void __launch_bounds__(kMvBlock) k_hop_counts(MvConst c, const uint64_t* __restrict__ rep,
-                                                         const uint32_t* __restrict__ table,
+                                                         const TE* __restrict__ table,
                                                          unsigned long long* __restrict__ partial) {
     const MvLds<double> L = stage_mv<double>(c, kMvBlock);
     __shared__ unsigned long long red[2][kMvBlock / kWave];
@@ -173,8 +174,8 @@ __global__ void __launch_bounds__(kMvBlock) k_hop_counts(MvConst c, const uint64
                 const int b = L.tto[t];
                 if ((m >> b) & 1) continue;
                 ++ng;
-                const uint32_t e = table[colex_rank<M>(mp | ((M)1 << b), L.bin, c.bs)];
-                if (e != kNoRow) ++ns;
+                const TE e = table[colex_rank<M>(mp | ((M)1 << b), L.bin, c.bs)];
+                if (e != ~(TE)0) ++ns;
             }
         }
     }
@@ -321,7 +322,7 @@ static bool flat_ranks_exact(const sed_sector* sec) {
     return cat->binom[(size_t)(cat->n_site - 1) * cat->bstride + cat->n_filled + 1] != 0xffffffffu;
 }
 
-template <typename M, typename R, bool F, int LPR, int B, int BLOCK, bool FLAT = false>
+template <typename M, typename R, bool F, typename TE, int LPR, int B, int BLOCK, bool FLAT = false>
 static int launch_shape(const MvConst& c, sed_sector* sec, const typename C2<R>::T* xh, typename C2<R>::T* y,
                         int chunk, hipStream_t s) {
     size_t off[9];
@@ -332,30 +333,41 @@ static int launch_shape(const MvConst& c, sed_sector* sec, const typename C2<R>:
     const int64_t nb = (sec->dim + rpb - 1) / rpb;
     const int64_t ch = chunk > 0 ? chunk : std::max<int64_t>(1, (nb + 7) / 8);
     const int64_t grid = ((nb + 8 * ch - 1) / (8 * ch)) * (8 * ch);
-    hipLaunchKernelGGL((k_matvec<M, R, F, LPR, B, BLOCK, FLAT>), dim3((unsigned)grid), dim3(BLOCK), lds, s, c,
-                       sec->d_rep.p, sec->d_stab.p, sec->d_table.p, xh, y, nb, ch);
+    const TE* table;
+    if constexpr (sizeof(TE) == 8) table = sec->d_table64.p;
+    else table = sec->d_table.p;
+    hipLaunchKernelGGL((k_matvec<M, R, F, LPR, B, BLOCK, FLAT, TE>), dim3((unsigned)grid), dim3(BLOCK), lds, s, c,
+                       sec->d_rep.p, sec->d_stab.p, table, xh, y, nb, ch);
     SED_HIP(hipGetLastError());
     return SED_OK;
 }
 
-template <typename M, typename R, bool F>
+// 8-byte-table sectors (TE = uint64_t) run the site-by-site shapes only.
+template <typename M, typename R, bool F, typename TE>
 static int run_shape(sed_sector* sec, int shape, int chunk, const typename C2<R>::T* xh, typename C2<R>::T* y,
                      hipStream_t s) {
     MvConst c = make_const<R>(sec);
     switch (shape) {
-        case 0: return launch_shape<M, R, F, 1, 1, 256>(c, sec, xh, y, chunk, s);
-        case 1: return launch_shape<M, R, F, 1, 4, 256>(c, sec, xh, y, chunk, s);
-        case 2: return launch_shape<M, R, F, 2, 4, 256>(c, sec, xh, y, chunk, s);
-        case 3: return launch_shape<M, R, F, 4, 4, 256>(c, sec, xh, y, chunk, s);
-        case 4: return launch_shape<M, R, F, 8, 4, 256>(c, sec, xh, y, chunk, s);
-        case 5: return launch_shape<M, R, F, 4, 2, 256>(c, sec, xh, y, chunk, s);
-        case 6: return launch_shape<M, R, F, 2, 4, 128>(c, sec, xh, y, chunk, s);
-        case 7: return launch_shape<M, R, F, 1, 4, 512>(c, sec, xh, y, chunk, s);
-        case 8: return launch_shape<M, R, F, 2, 4, 512>(c, sec, xh, y, chunk, s);
-        case 9: return launch_shape<M, R, F, 1, 4, 256, true>(c, sec, xh, y, chunk, s);
-        case 10: return launch_shape<M, R, F, 1, 8, 256, true>(c, sec, xh, y, chunk, s);
-        case 11: return launch_shape<M, R, F, 2, 4, 256, true>(c, sec, xh, y, chunk, s);
-        default: return launch_shape<M, R, F, 2, 8, 256, true>(c, sec, xh, y, chunk, s);
+        case 0: return launch_shape<M, R, F, TE, 1, 1, 256>(c, sec, xh, y, chunk, s);
+        case 1: return launch_shape<M, R, F, TE, 1, 4, 256>(c, sec, xh, y, chunk, s);
+        case 2: return launch_shape<M, R, F, TE, 2, 4, 256>(c, sec, xh, y, chunk, s);
+        case 3: return launch_shape<M, R, F, TE, 4, 4, 256>(c, sec, xh, y, chunk, s);
+        case 4: return launch_shape<M, R, F, TE, 8, 4, 256>(c, sec, xh, y, chunk, s);
+        case 5: return launch_shape<M, R, F, TE, 4, 2, 256>(c, sec, xh, y, chunk, s);
+        case 6: return launch_shape<M, R, F, TE, 2, 4, 128>(c, sec, xh, y, chunk, s);
+        case 7: return launch_shape<M, R, F, TE, 1, 4, 512>(c, sec, xh, y, chunk, s);
+        case 8: return launch_shape<M, R, F, TE, 2, 4, 512>(c, sec, xh, y, chunk, s);
+        default: break;
+    }
+    if constexpr (sizeof(TE) == 8) {
+        return fail(SED_EUNSUPPORTED, "flat hop loop on an 8-byte sector table");
+    } else {
+        switch (shape) {
+            case 9: return launch_shape<M, R, F, TE, 1, 4, 256, true>(c, sec, xh, y, chunk, s);
+            case 10: return launch_shape<M, R, F, TE, 1, 8, 256, true>(c, sec, xh, y, chunk, s);
+            case 11: return launch_shape<M, R, F, TE, 2, 4, 256, true>(c, sec, xh, y, chunk, s);
+            default: return launch_shape<M, R, F, TE, 2, 8, 256, true>(c, sec, xh, y, chunk, s);
+        }
     }
 }
 
@@ -364,7 +376,7 @@ template <typename R>
 static bool shape_fits(const sed_sector* sec, int shape) {
     MvConst c = make_const<R>(sec);
     size_t off[9];
-    if (kShapes[shape].flat && !flat_ranks_exact(sec)) return false;
+    if (kShapes[shape].flat && (sec->wide || !flat_ranks_exact(sec))) return false;
     return (int)mv_lds_layout<R>(c, off, kShapes[shape].block, kShapes[shape].flat, !kShapes[shape].flat) <=
            sec->cat->ctx->max_lds;
 }
@@ -374,10 +386,13 @@ static int run_any_shape(sed_sector* sec, int shape, int chunk, const typename C
                          hipStream_t s) {
     const bool F = sec->cat->stats == SED_FERMIONIC;
     if (sec->cat->n_site <= 32)
-        return F ? run_shape<uint32_t, R, true>(sec, shape, chunk, xh, y, s)
-                 : run_shape<uint32_t, R, false>(sec, shape, chunk, xh, y, s);
-    return F ? run_shape<uint64_t, R, true>(sec, shape, chunk, xh, y, s)
-             : run_shape<uint64_t, R, false>(sec, shape, chunk, xh, y, s);
+        return F ? run_shape<uint32_t, R, true, uint32_t>(sec, shape, chunk, xh, y, s)
+                 : run_shape<uint32_t, R, false, uint32_t>(sec, shape, chunk, xh, y, s);
+    if (sec->wide)
+        return F ? run_shape<uint64_t, R, true, uint64_t>(sec, shape, chunk, xh, y, s)
+                 : run_shape<uint64_t, R, false, uint64_t>(sec, shape, chunk, xh, y, s);
+    return F ? run_shape<uint64_t, R, true, uint32_t>(sec, shape, chunk, xh, y, s)
+             : run_shape<uint64_t, R, false, uint32_t>(sec, shape, chunk, xh, y, s);
 }
 
 // Autotuner: on the first matvec of a sector large enough for the launch shape to
@@ -578,8 +593,8 @@ int launch_gauge_rows(sed_sector* sec, const double* f_dev) {
     return SED_OK;
 }
 
-template <typename M>
-static int run_counts(sed_sector* sec, int64_t* gated, int64_t* ins) {
+template <typename M, typename TE>
+static int run_counts(sed_sector* sec, const TE* table, int64_t* gated, int64_t* ins) {
     hipStream_t s = sec->cat->ctx->stream;
     MvConst c = make_const<double>(sec);
     size_t off[9];
@@ -587,8 +602,8 @@ static int run_counts(sed_sector* sec, int64_t* gated, int64_t* ins) {
     const int64_t nb = std::max<int64_t>(1, (sec->dim + kMvBlock - 1) / kMvBlock);
     DevBuf<unsigned long long> part;
     SED_TRY(part.alloc(2 * nb));
-    hipLaunchKernelGGL(k_hop_counts<M>, dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p,
-                       sec->d_table.p, part.p);
+    hipLaunchKernelGGL((k_hop_counts<M, TE>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, table,
+                       part.p);
     SED_HIP(hipGetLastError());
     std::vector<unsigned long long> h(2 * nb);
     SED_HIP(hipMemcpyAsync(h.data(), part.p, part.bytes(), hipMemcpyDeviceToHost, s));
@@ -608,11 +623,13 @@ int launch_hop_counts(sed_sector* sec, int64_t* gated, int64_t* in_sector) {
         *gated = *in_sector = 0;
         return SED_OK;
     }
-    if (sec->cat->n_site <= 32) return run_counts<uint32_t>(sec, gated, in_sector);
-    return run_counts<uint64_t>(sec, gated, in_sector);
+    if (sec->cat->n_site <= 32) return run_counts<uint32_t>(sec, sec->d_table.p, gated, in_sector);
+    if (sec->wide) return run_counts<uint64_t>(sec, sec->d_table64.p, gated, in_sector);
+    return run_counts<uint64_t>(sec, sec->d_table.p, gated, in_sector);
 }
 
 int launch_dense(sed_sector* sec, double* H_dev, hipStream_t s) {
+    if (sec->wide) return fail(SED_EUNSUPPORTED, "dense H' of an 8-byte-table sector");
     MvConst c = make_const<double>(sec);
     size_t off[9];
     const size_t lds = mv_lds_layout<double>(c, off, kMvBlock);

@@ -141,9 +141,9 @@ __device__ __forceinline__ MvLds<R> stage_mv(const MvConst& c, int block, bool f
 //     S[j] = sum_{i<j} (C(q_i, i+1) - C(q_i, i+2))    (uint32, modulo 2^32),
 // one pass over mp's bits per site.  Offsets into the binomial table are 24-bit
 // products (site < 64, stride < 70).
-template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST>
+template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST, typename TE>
 __device__ __forceinline__ void row_sums(const MvLds<R>& L, const MvConst& c, const M m, const int sub,
-                                         uint32_t* S, const uint32_t* __restrict__ table,
+                                         uint32_t* S, const TE* __restrict__ table,
                                          const typename C2<R>::T* __restrict__ xhat, R& ax, R& ay, R& dx,
                                          R& dy) {
     using T = typename C2<R>::T;
@@ -195,22 +195,23 @@ __device__ __forceinline__ void row_sums(const MvLds<R>& L, const MvConst& c, co
                 idx[u] = base + S[ib * SST] + L.bin[__umul24(b, c.bs) + ib + 1];
                 pj[u] = FERMI ? ((ib - ia) & 1) : 0;
             }
-            uint32_t e[BATCH];
+            constexpr TE none = ~(TE)0;
+            TE e[BATCH];
 #pragma unroll
             for (int u = 0; u < BATCH; ++u) {
-                const uint32_t v = table[ok[u] ? idx[u] : 0u];
-                e[u] = ok[u] ? v : kNoRow;
+                const TE v = table[ok[u] ? idx[u] : 0u];
+                e[u] = ok[u] ? v : none;
             }
             T v[BATCH];
 #pragma unroll
-            for (int u = 0; u < BATCH; ++u) v[u] = xhat[e[u] != kNoRow ? (e[u] >> c.shift) : 0u];
+            for (int u = 0; u < BATCH; ++u) v[u] = xhat[e[u] != none ? (e[u] >> c.shift) : (TE)0];
 #pragma unroll
             for (int u = 0; u < BATCH; ++u) {
-                const bool live = e[u] != kNoRow;
-                const int g = live ? (int)((e[u] >> 1) & c.gmask) : 0;
+                const bool live = e[u] != none;
+                const int g = live ? (int)(((uint32_t)e[u] >> 1) & c.gmask) : 0;
                 const T cf = cmul(L.tcoef[min(t + u, t1 - 1)], L.chi[g]);
                 const T pr = cmul(cf, v[u]);
-                const R sg = live ? ((((e[u] & 1) ^ (uint32_t)pj[u]) != 0) ? (R)-1 : (R)1) : (R)0;
+                const R sg = live ? (((((uint32_t)e[u] & 1) ^ (uint32_t)pj[u]) != 0) ? (R)-1 : (R)1) : (R)0;
                 ax += sg * pr.x;
                 ay += sg * pr.y;
             }
@@ -225,9 +226,9 @@ __device__ __forceinline__ void row_sums(const MvLds<R>& L, const MvConst& c, co
 // selects the signed character chi2[g << 1 | sign]; hops that are gated out or leave the
 // sector select chi2[1 << shift] = 0.  Each hop then adds conj(t') chi2 xhat[c] with four
 // fused multiply-adds.
-template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST>
+template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST, typename TE>
 __device__ __forceinline__ void row_sums_padded(const MvLds<R>& L, const MvConst& c, const M m, const int sub,
-                                                uint32_t* S, const uint32_t* __restrict__ table,
+                                                uint32_t* S, const TE* __restrict__ table,
                                                 const typename C2<R>::T* __restrict__ xhat, R& ax, R& ay, R& dx,
                                                 R& dy) {
     using T = typename C2<R>::T;
@@ -278,18 +279,19 @@ __device__ __forceinline__ void row_sums_padded(const MvLds<R>& L, const MvConst
                 idx[u] = base + S[ib * SST] + L.bin[(info >> 8) + ib + 1];
                 pj[u] = FERMI ? (uint32_t)((ib ^ ia) & 1) : 0u;
             }
-            uint32_t e[BATCH];
+            constexpr TE none = ~(TE)0;
+            TE e[BATCH];
 #pragma unroll
             for (int u = 0; u < BATCH; ++u) {
-                const uint32_t v = table[ok[u] ? idx[u] : 0u];
-                e[u] = ok[u] ? v : kNoRow;
+                const TE v = table[ok[u] ? idx[u] : 0u];
+                e[u] = ok[u] ? v : none;
             }
             T v[BATCH];
 #pragma unroll
-            for (int u = 0; u < BATCH; ++u) v[u] = xhat[e[u] != kNoRow ? (e[u] >> c.shift) : 0u];
+            for (int u = 0; u < BATCH; ++u) v[u] = xhat[e[u] != none ? (e[u] >> c.shift) : (TE)0];
 #pragma unroll
             for (int u = 0; u < BATCH; ++u) {
-                const uint32_t ci = e[u] != kNoRow ? ((e[u] ^ pj[u]) & mask2) : dead;
+                const uint32_t ci = e[u] != none ? ((uint32_t)e[u] ^ pj[u]) & mask2 : dead;
                 const T cf = cmul(L.tcoef[t + u], L.chi[ci]);
                 ax += cf.x * v[u].x;
                 ax -= cf.y * v[u].y;
@@ -311,9 +313,9 @@ __device__ __forceinline__ void row_sums_padded(const MvLds<R>& L, const MvConst
 // (the host enables this loop only when C(n-1, k+1) < 2^32).  The lane's candidates
 // (sites sub, sub+LPR, ... and their term lists, in order) are walked in batches of
 // BATCH that run across site boundaries: BATCH table reads, then BATCH gathers.
-template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST>
+template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST, typename TE>
 __device__ __forceinline__ void row_sums_flat(const MvLds<R>& L, const MvConst& c, const M m, const int sub,
-                                              uint32_t* P, const uint32_t* __restrict__ table,
+                                              uint32_t* P, const TE* __restrict__ table,
                                               const typename C2<R>::T* __restrict__ xhat, R& ax, R& ay, R& dx,
                                               R& dy) {
     using T = typename C2<R>::T;
@@ -386,22 +388,23 @@ __device__ __forceinline__ void row_sums_flat(const MvLds<R>& L, const MvConst& 
             pj[u] = FERMI ? ((ib - ia) & 1) : 0;
             if (live) ++t;
         }
-        uint32_t e[BATCH];
+        constexpr TE none = ~(TE)0;
+        TE e[BATCH];
 #pragma unroll
         for (int u = 0; u < BATCH; ++u) {
-            const uint32_t v = table[ok[u] ? idx[u] : 0u];
-            e[u] = ok[u] ? v : kNoRow;
+            const TE v = table[ok[u] ? idx[u] : 0u];
+            e[u] = ok[u] ? v : none;
         }
         T v[BATCH];
 #pragma unroll
-        for (int u = 0; u < BATCH; ++u) v[u] = xhat[e[u] != kNoRow ? (e[u] >> c.shift) : 0u];
+        for (int u = 0; u < BATCH; ++u) v[u] = xhat[e[u] != none ? (e[u] >> c.shift) : (TE)0];
 #pragma unroll
         for (int u = 0; u < BATCH; ++u) {
-            const bool live = e[u] != kNoRow;
-            const int g = live ? (int)((e[u] >> 1) & c.gmask) : 0;
+            const bool live = e[u] != none;
+            const int g = live ? (int)(((uint32_t)e[u] >> 1) & c.gmask) : 0;
             const T cf = cmul(L.tcoef[tt[u]], L.chi[g]);
             const T pr = cmul(cf, v[u]);
-            const R sg = live ? ((((e[u] & 1) ^ (uint32_t)pj[u]) != 0) ? (R)-1 : (R)1) : (R)0;
+            const R sg = live ? (((((uint32_t)e[u] & 1) ^ (uint32_t)pj[u]) != 0) ? (R)-1 : (R)1) : (R)0;
             ax += sg * pr.x;
             ay += sg * pr.y;
         }

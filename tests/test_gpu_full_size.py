@@ -78,7 +78,7 @@ class FullSector:
 
 
 FULL = {"checker4x6": lambda: L.workload("checker4x6"), "hubbard4x4": lambda: L.workload("hubbard4x4"),
-        "xxz32": lambda: L.heisenberg_chain(32)}
+        "xxz32": lambda: L.heisenberg_chain(32), "xxz34": lambda: L.heisenberg_chain(34)}
 
 
 @pytest.fixture(scope="module")
@@ -100,8 +100,10 @@ def full(ed, ctx):
         v.close()
 
 
+# XXZ N=34 at half filling: C(34, 17) = 2,333,606,220 ranks (above 2^31) and a k = 0
+# sector of 68,635,478 rows, which takes the 8-byte sector table
 FULL_CASES = [("checker4x6", (0.0, 0.0)), ("checker4x6", (0.3, 0.7)), ("hubbard4x4", (0.0, 0.0)),
-              ("xxz32", (0.0, 0.0))]
+              ("xxz32", (0.0, 0.0)), ("xxz34", (0.0, 0.0))]
 
 
 @pytest.mark.parametrize("name,theta", FULL_CASES)
@@ -271,4 +273,29 @@ def test_hubbard4x4_fp32_refine_matches_fp64(ed, ctx):
     v32 = ed.diagonalize_block_matrixfree(H, n, nev=3, precision=ed.PREC_FP32_REFINE, return_vectors=False)[0]
     assert np.abs(v32 - v64).max() <= 1e-10 * np.abs(v64).max(), (v32, v64)
     b.close()
+    cat.close()
+
+
+def test_xxz34_catalog_above_2e31_ranks_and_every_sector_dimension(ed, ctx):
+    """XXZ N=34 at half filling: C(34, 17) = 2,333,606,220 ranks, above the int32 rank range
+    (the catalog stores only representative flags in its first pass).  Every momentum
+    sector's dimension equals the character formula, the sectors partition the ranks, and
+    the orbit count equals the number of necklaces (Burnside over the 34 translations)."""
+    m = L.heisenberg_chain(34)
+    G = L.model_group(m)
+    print(f"[xxz34] catalog N=34 k=17 |G|={G.order}", flush=True)
+    cat = catalog(ed, ctx, m, G)
+    assert cat.n_total == math.comb(34, 17) == 2333606220
+    necklaces = sum(math.comb(math.gcd(d, 34), 17 * math.gcd(d, 34) // 34) for d in range(34)
+                    if (17 * math.gcd(d, 34)) % 34 == 0) // 34
+    assert cat.n_orbits == necklaces
+    total = 0
+    for ir in L.translation_irreps(G, m.lattice):
+        b = ed.build_symmetry_sector_basis(cat, ir)
+        want = round(character_dim(m, G, ir.values))
+        print(f"[xxz34] sector {ir.label} dim={b.dim} formula={want}", flush=True)
+        assert b.dim == want, ir.label
+        total += b.dim
+        b.close()
+    assert total == cat.n_total
     cat.close()
