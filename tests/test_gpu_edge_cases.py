@@ -143,11 +143,17 @@ def test_error_codes_and_recovery(ed, ctx):
     code = _err(lambda: ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=4, symmetry_group=bad,
                                                         particle_statistics=m.statistics, ctx=ctx))
     assert code in (_lib.SED_EUNSUPPORTED, _lib.SED_EINVAL)
-    # C(N, k) beyond 2^31 ranks
-    big = L.heisenberg_chain(40)
-    assert _err(lambda: ed.build_symmetry_orbit_catalog(n_site=40, n_filled=20, symmetry_group=L.model_group(big),
+    # C(N, k) beyond 2^32 - 2 ranks: C(36, 18) = 9,075,135,300
+    big = L.heisenberg_chain(36)
+    assert _err(lambda: ed.build_symmetry_orbit_catalog(n_site=36, n_filled=18, symmetry_group=L.model_group(big),
                                                         particle_statistics=big.statistics, ctx=ctx)) \
         == _lib.SED_EUNSUPPORTED
+    assert "2^32" in lib.sed_last_error().decode()
+    # more than 2^31 orbits: the identity group on 34 sites at half filling (C(34, 17) orbits)
+    assert _err(lambda: ed.build_symmetry_orbit_catalog(n_site=34, n_filled=17, symmetry_group=L.identity_group(34),
+                                                        particle_statistics=big.statistics, ctx=ctx)) \
+        == _lib.SED_EUNSUPPORTED
+    assert "orbits exceed 2^31" in lib.sed_last_error().decode()
     # terms: out-of-range site, i_from == i_to
     gc = ed.build_symmetry_orbit_catalog(n_site=m.n_site, n_filled=4, symmetry_group=G,
                                          particle_statistics=m.statistics, ctx=ctx)

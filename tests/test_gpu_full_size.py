@@ -168,7 +168,7 @@ def test_full_size_every_launch_shape_matches_the_oracle(ed, oracle_mod, full):
     assert ran >= 2 * 9
 
 
-@pytest.mark.parametrize("name", ["checker4x6", "hubbard4x4"])
+@pytest.mark.parametrize("name", ["checker4x6", "hubbard4x4", "xxz34"])
 def test_full_size_eigenpairs_satisfy_the_oracle_rows(ed, oracle_mod, full, name):
     """The lowest three eigenpairs of the k = (0,0) sector, FP64 and FP32+refine: on the
     sampled rows the oracle's (H v)[r] equals lambda v[r] (2e-10, the solver's residual
@@ -192,6 +192,40 @@ def test_full_size_eigenpairs_satisfy_the_oracle_rows(ed, oracle_mod, full, name
     ref = out[(ed.PREC_FP64, 1234)]
     for v in out.values():
         assert np.abs(v - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_xxz34_every_launch_shape_on_the_8_byte_table(ed, oracle_mod, full):
+    """The k = 0 sector of XXZ N=34 (68,635,478 rows at |G| = 34) does not fit the 4-byte
+    table entry (row < 2^25 beside 7 group and parity bits) and runs on the 8-byte table:
+    every site-by-site launch shape against the oracle's rows, FP64 and FP32; the flat hop
+    loop is refused with SED_EUNSUPPORTED."""
+    fs = full("xxz34", (0.0, 0.0))
+    assert fs.b.dim == 68635478 and fs.b.dim >= 1 << 25
+    assert fs.check_basis(oracle_mod) == ((0, -1), True)
+    rng = np.random.default_rng(29)
+    x = rng.standard_normal(fs.b.dim) + 1j * rng.standard_normal(fs.b.dim)
+    rows = fs.rows(4)
+    ref = fs.oracle_rows(oracle_mod, x, rows)
+    scale = np.abs(ref).max()
+    n_shapes = len(fs.b.shape(ed.PREC_FP64)["tune_ms"])
+    ran, refused = 0, 0
+    for prec, tol in ((ed.PREC_FP64, 1e-12), (ed.PREC_FP32, 5e-6)):
+        fs.H.precision = prec
+        for sh in range(n_shapes):
+            fs.b.set_shape(prec, sh, 0)
+            try:
+                y = fs.H(x)
+            except _lib.SedError as e:
+                assert e.code == _lib.SED_EUNSUPPORTED, (sh, e)
+                refused += 1
+                continue
+            err = np.abs(y[rows] - ref).max() / scale
+            print(f"[xxz34-shape] prec={prec} shape={sh} rel_err={err:.2e}", flush=True)
+            assert err <= tol, (prec, sh)
+            ran += 1
+        fs.b.set_shape(prec, -1)
+    fs.H.precision = ed.PREC_FP64
+    assert ran == 2 * 9 and refused == 2 * (n_shapes - 9)
 
 
 def catalog(ed, ctx, m, G):
