@@ -77,9 +77,13 @@ int sed_destroy(sed_context* ctx);
 
 /* build_symmetry_orbit_catalog(; n_filled, symmetry_group, particle_statistics)
  * (:371-487) together with Finite_Symmetry_Group (:90-104) and
- * _inverse_operation_indices (:109-117).  The site phases must be of gauge form
- * eta_g(i) = f(i)/f(perm_g(i)) (true of build_translation_group at any twist,
- * :1374-1405); otherwise SED_EUNSUPPORTED.  Requires n_site <= 64,
+ * _inverse_operation_indices (:109-117).  Site phases of gauge form
+ * eta_g(i) = f(i)/f(perm_g(i)) (build_translation_group at any twist, :1374-1405) run on
+ * sign-only tables.  Other site phases are accepted when the group's operators form a
+ * representation (the permutations are closed under composition and
+ * eta_{g o h}(i) = eta_h(i) eta_g(perm_h(i)), e.g. a phase on a site that rotations
+ * fix): the catalog then also keeps a per-rank hop amplitude (24 B per rank) and H|psi>
+ * runs one launch shape; phases that are neither give SED_EUNSUPPORTED.  Requires n_site <= 64,
  * C(n_site, n_filled) <= 2^32 - 2 (colex ranks are 32-bit), at most 2^31 orbits and
  * n_group <= 64.  Sectors built on it hold at most 2^31 - 1 rows. */
 int sed_catalog_build(sed_context* ctx, int n_site, int n_filled, int statistics,
@@ -101,10 +105,11 @@ int sed_catalog_stabilizers(const sed_catalog* cat, int32_t* stab_gidx, double* 
  * another flux twist) without re-enumerating orbits: the orbit tables depend on the
  * permutations only, and with phases of gauge form f(i)/f(perm_g(i)) a stabilizer
  * element's phase on its representative is its reordering sign, so sector membership
- * does not change either.  Later dumps (sed_catalog_copy, sed_catalog_stabilizers) and
- * sectors built afterwards use the new phases; sectors built before keep theirs.
- * SED_EUNSUPPORTED (catalog unchanged) when the phases are not of gauge form or would
- * change the inverse elements. */
+ * does not change either (with other phases the per-rank hop amplitudes are recomputed
+ * and membership follows the new stabilizer phases).  Later dumps (sed_catalog_copy,
+ * sed_catalog_stabilizers) and sectors built afterwards use the new phases; sectors built
+ * before keep theirs.  SED_EUNSUPPORTED (catalog unchanged) when the phases are neither of
+ * gauge form nor a representation, or would change the inverse elements. */
 int sed_catalog_update_phases(sed_catalog* cat, const double* phases);
 int sed_catalog_destroy(sed_catalog* cat);
 

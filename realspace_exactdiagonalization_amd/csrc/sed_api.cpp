@@ -178,11 +178,40 @@ static int set_group_phases(sed_catalog* cat, const double* phases) {
             }
         }
     }
-    for (int g = 0; g < n_group; ++g)
+    cat->general = false;
+    for (int g = 0; g < n_group && !cat->general; ++g)
         for (int i = 0; i < n_site; ++i) {
             const int t = cat->perm[(size_t)g * n_site + i] - 1;
-            if (std::abs(cat->phases[(size_t)g * n_site + i] * cat->gauge_f[t] - cat->gauge_f[i]) > 1e-10)
-                return fail(SED_EUNSUPPORTED, "site phases are not of gauge form f(i)/f(perm(i))");
+            if (std::abs(cat->phases[(size_t)g * n_site + i] * cat->gauge_f[t] - cat->gauge_f[i]) > 1e-10) {
+                cat->general = true;
+                break;
+            }
+        }
+    if (!cat->general) return SED_OK;
+    // Not of gauge form: accepted when the operators U_g|s> = alpha_g(s)|g(s)> form a
+    // representation, U_g U_h = U_{g o h}, i.e. for every pair the composed permutation is
+    // an element k of the group and eta_k(i) = eta_h(i) eta_g(perm_h(i)) (a cocycle);
+    // only then are the projected sector states and the reference's sector matrix what
+    // they are meant to be.
+    cat->gauge_f.assign(n_site, cd(1.0, 0.0));
+    std::vector<int> comp(n_site);
+    for (int g = 0; g < n_group; ++g)
+        for (int h = 0; h < n_group; ++h) {
+            for (int i = 0; i < n_site; ++i)
+                comp[i] = cat->perm[(size_t)g * n_site + cat->perm[(size_t)h * n_site + i] - 1];
+            int k = -1;
+            for (int c = 0; c < n_group && k < 0; ++c)
+                if (std::equal(comp.begin(), comp.end(), cat->perm.begin() + (size_t)c * n_site)) k = c;
+            if (k < 0)
+                return fail(SED_EUNSUPPORTED, "site phases not of gauge form f(i)/f(perm(i)) on permutations "
+                                              "that are not closed under composition");
+            for (int i = 0; i < n_site; ++i) {
+                const int hi = cat->perm[(size_t)h * n_site + i] - 1;
+                const cd want = cat->phases[(size_t)h * n_site + i] * cat->phases[(size_t)g * n_site + hi];
+                if (std::abs(cat->phases[(size_t)k * n_site + i] - want) > 1e-10)
+                    return fail(SED_EUNSUPPORTED, "site phases are neither of gauge form f(i)/f(perm(i)) nor a "
+                                                  "representation of the group (eta_gh(i) != eta_h(i) eta_g(h(i)))");
+            }
         }
     return SED_OK;
 }
@@ -250,6 +279,10 @@ int sed_catalog_build(sed_context* ctx, int n_site, int n_filled, int statistics
     SED_TRY(upload(cat->d_perm, p8.data(), p8.size(), s));
     SED_TRY(upload(cat->d_inv, inv8.data(), inv8.size(), s));
     SED_TRY(launch_catalog(cat));
+    if (cat->general) {
+        cat->amp = std::make_shared<HopAmp>();
+        SED_TRY(launch_hop_amp(cat, cat->amp.get()));
+    }
     *out = guard.release();
     context_child_added(ctx);
     return SED_OK;
@@ -346,19 +379,28 @@ int sed_catalog_stabilizers(const sed_catalog* cat, int32_t* stab_gidx, double* 
 
 int sed_catalog_update_phases(sed_catalog* cat, const double* phases) {
     if (!cat) return fail(SED_EINVAL, "NULL catalog");
+    SED_HIP(hipSetDevice(cat->ctx->device));
     const std::vector<cd> old_phases = cat->phases, old_f = cat->gauge_f;
     const std::vector<int32_t> old_inv = cat->inv;
-    const bool old_has = cat->has_phases;
+    const bool old_has = cat->has_phases, old_general = cat->general;
     int rc = set_group_phases(cat, phases);
     if (rc == SED_OK && cat->inv != old_inv)
         rc = fail(SED_EUNSUPPORTED, "the new phases change the inverse group elements (rebuild the catalog)");
+    std::shared_ptr<HopAmp> amp;   // sectors built before keep the table of their phases
+    if (rc == SED_OK && cat->general) {
+        amp = std::make_shared<HopAmp>();
+        rc = launch_hop_amp(cat, amp.get());
+    }
     if (rc != SED_OK) {
         cat->phases = old_phases;
         cat->gauge_f = old_f;
         cat->inv = old_inv;
         cat->has_phases = old_has;
+        cat->general = old_general;
+        return rc;
     }
-    return rc;
+    cat->amp = amp;
+    return SED_OK;
 }
 
 int sed_catalog_destroy(sed_catalog* cat) {
@@ -402,7 +444,7 @@ int sed_sector_build(sed_catalog* cat, const double* chi, double atol, sed_secto
         SED_TRY(upload(sec->d_chi2_32, c2f.data(), c2f.size(), s));
         SED_HIP(hipStreamSynchronize(s));   // the host arrays end with this scope
     }
-    sec->has_gauge = cat->has_phases;
+    sec->has_gauge = cat->has_phases && !cat->general;
     sec->gauge_f = cat->gauge_f;
     sec->phases = cat->phases;
     if (sec->has_gauge && sec->dim > 0) {
@@ -827,6 +869,7 @@ int sed_sector_one_body(sed_sector* sec, int which, double* rho) {
     tmp.d_table.borrow(sec->d_table);
     tmp.wide = sec->wide;
     tmp.d_table64.borrow(sec->d_table64);
+    tmp.amp = sec->amp;
     tmp.d_chi64.borrow(sec->d_chi64);
     tmp.d_chi32.borrow(sec->d_chi32);
     tmp.d_chi2_64.borrow(sec->d_chi2_64);

@@ -406,6 +406,63 @@ static int run_amp(const sed_catalog* cat, double* d_amp) {
     return SED_OK;
 }
 
+// Per-rank hop amplitude of a group with general site phases: for rank r (mask s in the
+// orbit of rep, stored element h with h(s) = rep, g = h^-1), prod_{i in rep} eta_g(i) in
+// ascending site order (the reordering sign lives in the sector table's parity bit): the
+// conjugate of the reference's rep_amp_table entry (src/symmetry_resolved_ed.jl:466)
+// without its sign.
+template <typename M>
+__global__ void __launch_bounds__(kCatBlock) k_cat_hop_amp(CatConst c, const double2* __restrict__ phases,
+                                                           int64_t n_total, const int32_t* __restrict__ rr,
+                                                           const uint8_t* __restrict__ sym_par,
+                                                           const uint64_t* __restrict__ orep,
+                                                           double2* __restrict__ a64, float2* __restrict__ a32) {
+    uint32_t* bin;
+    uint8_t *perm, *inv;
+    stage_cat(c, bin, perm, inv);
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_total;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t o = -(int64_t)rr[r] - 1;
+        M rep = (M)orep[o];
+        const double2* ph = phases + (size_t)inv[sym_par[r] & 0x7f] * c.n;
+        double2 a = make_double2(1.0, 0.0);
+        while (rep) {
+            const int i = ctz_m<M>(rep);
+            rep &= rep - 1;
+            const double2 b = ph[i];
+            a = make_double2(__dsub_rn(__dmul_rn(a.x, b.x), __dmul_rn(a.y, b.y)),
+                             __dadd_rn(__dmul_rn(a.x, b.y), __dmul_rn(a.y, b.x)));
+        }
+        a64[r] = a;
+        a32[r] = make_float2((float)a.x, (float)a.y);
+    }
+}
+
+int launch_hop_amp(const sed_catalog* cat, HopAmp* out) {
+    hipStream_t s = cat->ctx->stream;
+    CatConst c{cat->d_binom.p, cat->d_perm.p, cat->d_inv.p, cat->n_site, cat->n_filled, cat->n_group,
+               cat->bstride};
+    DevBuf<double> ph;
+    SED_TRY(ph.alloc((size_t)2 * cat->n_group * cat->n_site));
+    SED_HIP(hipMemcpyAsync(ph.p, cat->phases.data(), ph.bytes(), hipMemcpyHostToDevice, s));
+    const int64_t n = cat->n_total;
+    SED_TRY(out->a64.alloc((size_t)2 * n));
+    SED_TRY(out->a32.alloc((size_t)2 * n));
+    const int64_t nblk = std::max<int64_t>(1, std::min<int64_t>((n + kCatBlock - 1) / kCatBlock, 1 << 22));
+    auto* a64 = reinterpret_cast<double2*>(out->a64.p);
+    auto* a32 = reinterpret_cast<float2*>(out->a32.p);
+    const auto* phd = reinterpret_cast<const double2*>(ph.p);
+    if (cat->n_site <= 32)
+        hipLaunchKernelGGL(k_cat_hop_amp<uint32_t>, dim3((unsigned)nblk), dim3(kCatBlock), cat_lds_bytes(cat), s, c,
+                           phd, n, cat->d_rep_rank.p, cat->d_sym_par.p, cat->d_orbit_rep.p, a64, a32);
+    else
+        hipLaunchKernelGGL(k_cat_hop_amp<uint64_t>, dim3((unsigned)nblk), dim3(kCatBlock), cat_lds_bytes(cat), s, c,
+                           phd, n, cat->d_rep_rank.p, cat->d_sym_par.p, cat->d_orbit_rep.p, a64, a32);
+    SED_HIP(hipGetLastError());
+    SED_HIP(hipStreamSynchronize(s));
+    return SED_OK;
+}
+
 int launch_catalog_amp(const sed_catalog* cat, double* d_amp) {
     const bool f = cat->stats == SED_FERMIONIC;
     if (cat->n_site <= 32) return f ? run_amp<uint32_t, true>(cat, d_amp) : run_amp<uint32_t, false>(cat, d_amp);
@@ -435,6 +492,37 @@ __global__ void __launch_bounds__(256) k_sec_flags(int64_t n_orbits, int nG, con
             st &= st - 1;
             const double target = ((sp >> h) & 1) ? -1.0 : 1.0;
             const double dx = lchi[h].x - target, dy = lchi[h].y;
+            if (!(sqrt(dx * dx + dy * dy) <= atol)) ok = 0;
+        }
+        flags[o] = ok;
+    }
+}
+
+// The same test with general site phases: alpha_h(rep) = sign * prod_{i in rep} eta_h(i)
+// (ascending site order, as apply_operation_to_mask multiplies them).
+__global__ void __launch_bounds__(256) k_sec_flags_general(int64_t n_orbits, int n, int nG,
+                                                           const double2* __restrict__ chi, double atol,
+                                                           const double2* __restrict__ phases,
+                                                           const uint64_t* __restrict__ orep,
+                                                           const uint64_t* __restrict__ ostab,
+                                                           const uint64_t* __restrict__ ostab_par,
+                                                           uint8_t* __restrict__ flags) {
+    __shared__ double2 lchi[64];
+    for (int i = threadIdx.x; i < nG; i += blockDim.x) lchi[i] = chi[i];
+    __syncthreads();
+    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n_orbits;
+         o += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t st = ostab[o];
+        const uint64_t sp = ostab_par[o];
+        const uint64_t rep = orep[o];
+        uint8_t ok = 1;
+        while (st) {
+            const int h = __builtin_ctzll(st);
+            st &= st - 1;
+            double2 a = make_double2(1.0, 0.0);
+            for (uint64_t t = rep; t; t &= t - 1) a = cmul(a, phases[(size_t)h * n + __builtin_ctzll(t)]);
+            if ((sp >> h) & 1) a = make_double2(-a.x, -a.y);
+            const double dx = lchi[h].x - a.x, dy = lchi[h].y - a.y;
             if (!(sqrt(dx * dx + dy * dy) <= atol)) ok = 0;
         }
         flags[o] = ok;
@@ -481,9 +569,20 @@ int launch_sector(sed_sector* sec, const double* chi_dev) {
     DevBuf<uint8_t> flags;
     SED_TRY(flags.alloc(no));
     int64_t nblk = std::max<int64_t>(1, std::min<int64_t>((no + 255) / 256, 1 << 22));
-    hipLaunchKernelGGL(k_sec_flags, dim3((unsigned)nblk), dim3(256), 0, s, no, cat->n_group,
-                       reinterpret_cast<const double2*>(chi_dev), sec->atol, cat->d_orbit_stab.p,
-                       cat->d_orbit_stab_par.p, flags.p);
+    DevBuf<double> ph;
+    if (cat->general) {
+        SED_TRY(ph.alloc((size_t)2 * cat->n_group * cat->n_site));
+        SED_HIP(hipMemcpyAsync(ph.p, cat->phases.data(), ph.bytes(), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_sec_flags_general, dim3((unsigned)nblk), dim3(256), 0, s, no, cat->n_site,
+                           cat->n_group, reinterpret_cast<const double2*>(chi_dev), sec->atol,
+                           reinterpret_cast<const double2*>(ph.p), cat->d_orbit_rep.p, cat->d_orbit_stab.p,
+                           cat->d_orbit_stab_par.p, flags.p);
+        sec->amp = cat->amp;
+    } else {
+        hipLaunchKernelGGL(k_sec_flags, dim3((unsigned)nblk), dim3(256), 0, s, no, cat->n_group,
+                           reinterpret_cast<const double2*>(chi_dev), sec->atol, cat->d_orbit_stab.p,
+                           cat->d_orbit_stab_par.p, flags.p);
+    }
     SED_HIP(hipGetLastError());
     ScanPlan plan;
     U8Flag fl{flags.p};
@@ -499,6 +598,9 @@ int launch_sector(sed_sector* sec, const double* chi_dev) {
     sec->wide = sec->dim > 0 && (uint64_t)sec->dim > ((1ull << (32 - sec->shift)) - 1);
     if (sec->wide && cat->n_site <= 32)   // C(32, k) / |G| always fits the 4-byte entry
         return fail(SED_EUNSUPPORTED, "8-byte sector table on a catalog of at most 32 sites");
+    if (sec->wide && cat->general)
+        return fail(SED_EUNSUPPORTED, "site phases not of gauge form on a sector of " + std::to_string(sec->dim) +
+                                          " rows (8-byte table)");
     DevBuf<int32_t> row_of;
     SED_TRY(row_of.alloc(no));
     SED_TRY(sec->d_rep.alloc(sec->dim));

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -91,6 +92,15 @@ struct DevBuf {
 
 struct Workspace;
 
+// Per-rank hop amplitudes of a group whose site phases are not of gauge form: for rank s
+// in the orbit of rep with stored element h (h(s) = rep), prod_{i in rep} eta_g(i) with
+// g = h^-1 (the phase part of the reference's conj(rep_amp_table[s])), FP64 and FP32.
+// Sectors keep the table of the phases they were built with (shared with the catalog).
+struct HopAmp {
+    DevBuf<double> a64;   // complex interleaved, n_total
+    DevBuf<float> a32;
+};
+
 }  // namespace sed
 
 struct sed_context {
@@ -111,8 +121,12 @@ struct sed_catalog {
     std::vector<int32_t> perm;        // 1-based, n_group x n_site
     std::vector<sed::cd> phases;      // n_group x n_site
     std::vector<int32_t> inv;         // 0-based inverse element index
-    std::vector<sed::cd> gauge_f;     // f(i): eta_g(i) = f(i)/f(perm_g(i))
+    std::vector<sed::cd> gauge_f;     // f(i): eta_g(i) = f(i)/f(perm_g(i)) (all 1 if general)
     bool has_phases = false;          // any eta != 1
+    // phases not of gauge form (a representation of the group with non-trivial phases on
+    // sites that some element fixes): hops take their amplitude from the per-rank table
+    bool general = false;
+    std::shared_ptr<sed::HopAmp> amp;  // general phases only
     std::vector<uint32_t> binom;      // (n_site+1) x bstride, saturating
     int bstride = 0;
     // device state
@@ -141,6 +155,7 @@ struct sed_sector {
     // same entry in 8 bytes: d_table64 (no row = all ones) and wide = true, d_table empty
     bool wide = false;
     sed::DevBuf<uint64_t> d_table64;
+    std::shared_ptr<sed::HopAmp> amp;  // the catalog's per-rank hop amplitudes (general phases)
     // terms
     bool has_terms = false;
     int n_terms = 0, n_dens = 0;
@@ -230,6 +245,7 @@ void context_child_removed(sed_context* ctx);
 // ---- kernel launchers (device code in the .hip files) ----
 int launch_catalog(sed_catalog* cat);
 int launch_catalog_amp(const sed_catalog* cat, double* d_amp);  // reference rep_amp_table on device
+int launch_hop_amp(const sed_catalog* cat, sed::HopAmp* out);    // per-rank hop amplitudes (general phases)
 int launch_sector(sed_sector* sec, const double* chi_dev);
 int launch_hop_counts(sed_sector* sec, int64_t* gated, int64_t* in_sector);
 

@@ -10,7 +10,9 @@
 // column's offset.  The values come out of the same sign-only tables as the H|psi>
 // kernel: H' (sign tables, gauge-transformed terms) has H'[c, r] = conj(H'[r, c]) =
 // t'_k conj(chi(g)) s sqrt(stab_c / stab_r), and with twisted boundaries
-// H[c, r] = conj(F(c)) H'[c, r] F(r), F(rep) = prod_{i in rep} f(i).
+// H[c, r] = conj(F(c)) H'[c, r] F(r), F(rep) = prod_{i in rep} f(i).  With site phases
+// not of gauge form (no F) each entry carries the conjugate of the target rank's hop
+// amplitude, H[c, r] = t_k conj(A) conj(chi(g)) s sqrt(stab_c / stab_r).
 #include <algorithm>
 #include <vector>
 
@@ -27,6 +29,7 @@ __global__ void __launch_bounds__(kMvBlock) k_csc_entries(MvConst c, const uint6
                                                           const uint8_t* __restrict__ stab,
                                                           const uint32_t* __restrict__ table,
                                                           const double2* __restrict__ F,
+                                                          const double2* __restrict__ amp,
                                                           const int64_t* __restrict__ off, int32_t* __restrict__ cnt,
                                                           int64_t* __restrict__ erow, double2* __restrict__ eval) {
     const MvLds<double> L = stage_mv<double>(c, kMvBlock);
@@ -54,7 +57,8 @@ __global__ void __launch_bounds__(kMvBlock) k_csc_entries(MvConst c, const uint6
             const int b = L.tto[t];
             if ((m >> b) & 1) continue;
             const M nm = mp | ((M)1 << b);
-            const uint32_t e = table[colex_rank<M>(nm, L.bin, c.bs)];
+            const uint32_t rank = colex_rank<M>(nm, L.bin, c.bs);
+            const uint32_t e = table[rank];
             if (e == kNoRow) continue;
             ++n;
             if (!FILL) continue;
@@ -63,7 +67,8 @@ __global__ void __launch_bounds__(kMvBlock) k_csc_entries(MvConst c, const uint6
             int p = e & 1;
             const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
             if (FERMI) p ^= (ib - ia) & 1;
-            const double2 cf = cmul(L.tcoef[t], L.chi[g]);          // conj(t') chi(g)
+            double2 cf = cmul(L.tcoef[t], L.chi[g]);                // conj(t') chi(g)
+            if (amp) cf = cmul(cf, amp[rank]);                      // times the hop amplitude
             const double sc = sqrt((double)stab[col]) * inv * (p ? -1.0 : 1.0);
             double2 v = make_double2(cf.x * sc, -cf.y * sc);         // H'[col, r]
             if (F) {
@@ -93,13 +98,14 @@ static int launch_entries(sed_sector* sec, const int64_t* off, int32_t* cnt, int
     const int64_t nb = (sec->dim + kMvBlock - 1) / kMvBlock;
     const bool F = sec->cat->stats == SED_FERMIONIC;
     const double2* G = sec->has_gauge ? reinterpret_cast<const double2*>(sec->d_gauge.p) : nullptr;
+    const double2* A = sec->amp ? reinterpret_cast<const double2*>(sec->amp->a64.p) : nullptr;
     const dim3 grid((unsigned)nb), blk(kMvBlock);
     if (sec->cat->n_site <= 32) {
-        if (F) hipLaunchKernelGGL((k_csc_entries<uint32_t, true, FILL>), grid, blk, lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, G, off, cnt, erow, eval);
-        else hipLaunchKernelGGL((k_csc_entries<uint32_t, false, FILL>), grid, blk, lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, G, off, cnt, erow, eval);
+        if (F) hipLaunchKernelGGL((k_csc_entries<uint32_t, true, FILL>), grid, blk, lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, G, A, off, cnt, erow, eval);
+        else hipLaunchKernelGGL((k_csc_entries<uint32_t, false, FILL>), grid, blk, lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, G, A, off, cnt, erow, eval);
     } else {
-        if (F) hipLaunchKernelGGL((k_csc_entries<uint64_t, true, FILL>), grid, blk, lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, G, off, cnt, erow, eval);
-        else hipLaunchKernelGGL((k_csc_entries<uint64_t, false, FILL>), grid, blk, lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, G, off, cnt, erow, eval);
+        if (F) hipLaunchKernelGGL((k_csc_entries<uint64_t, true, FILL>), grid, blk, lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, G, A, off, cnt, erow, eval);
+        else hipLaunchKernelGGL((k_csc_entries<uint64_t, false, FILL>), grid, blk, lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, G, A, off, cnt, erow, eval);
     }
     SED_HIP(hipGetLastError());
     return SED_OK;

@@ -44,14 +44,15 @@ namespace sed {
 // fixed xor butterfly over the row's lanes.  Hops are processed BATCH at a time: BATCH
 // independent table reads are issued before the BATCH dependent gathers, so each lane
 // keeps several memory requests in flight.
-template <typename M, typename R, bool FERMI, int LPR, int BATCH, int BLOCK, bool FLAT, typename TE>
+template <typename M, typename R, bool FERMI, int LPR, int BATCH, int BLOCK, bool FLAT, typename TE, bool AMP>
 __global__ void __launch_bounds__(BLOCK) k_matvec(MvConst c, const uint64_t* __restrict__ rep,
                                                   const uint8_t* __restrict__ stab,
                                                   const TE* __restrict__ table,
                                                   const typename C2<R>::T* __restrict__ xhat,
                                                   typename C2<R>::T* __restrict__ y, int64_t nblocks,
-                                                  int64_t chunk) {
+                                                  int64_t chunk, const typename C2<R>::T* __restrict__ amp) {
     using T = typename C2<R>::T;
+    static_assert(!(FLAT && AMP), "hop amplitudes run on the site-by-site loop only");
     constexpr int RPB = BLOCK / LPR;
     const MvLds<R> L = stage_mv<R>(c, BLOCK, FLAT, !FLAT);
     const int64_t lb = xcd_block(nblocks, chunk);
@@ -65,8 +66,8 @@ __global__ void __launch_bounds__(BLOCK) k_matvec(MvConst c, const uint64_t* __r
         row_sums_flat<M, R, FERMI, LPR, BATCH, BLOCK, TE>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay, dx,
                                                           dy);
     else
-        row_sums_padded<M, R, FERMI, LPR, BATCH, BLOCK, TE>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax, ay,
-                                                            dx, dy);
+        row_sums_padded<M, R, FERMI, LPR, BATCH, BLOCK, TE, AMP>(L, c, m, sub, L.S + threadIdx.x, table, xhat, ax,
+                                                                 ay, dx, dy, amp);
     if (LPR > 1) {
 #pragma unroll
         for (int off = LPR / 2; off >= 1; off >>= 1) {
@@ -111,7 +112,7 @@ template <typename M, bool FERMI>
 __global__ void __launch_bounds__(kMvBlock) k_dense(MvConst c, const uint64_t* __restrict__ rep,
                                                     const uint8_t* __restrict__ stab,
                                                     const uint32_t* __restrict__ table,
-                                                    double2* __restrict__ H) {
+                                                    const double2* __restrict__ amp, double2* __restrict__ H) {
     const MvLds<double> L = stage_mv<double>(c, kMvBlock);
     const int64_t r = (int64_t)blockIdx.x * kMvBlock + threadIdx.x;
     if (r >= c.dim) return;
@@ -144,6 +145,7 @@ __global__ void __launch_bounds__(kMvBlock) k_dense(MvConst c, const uint64_t* _
             const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
             if (FERMI) p ^= (ib - ia) & 1;
             double2 cf = cmul(L.tcoef[t], L.chi[g]);
+            if (amp) cf = cmul(cf, amp[rank]);
             const double sc = sqrt((double)stab[col]) * inv * (p ? -1.0 : 1.0);
             row[col].x += cf.x * sc;
             row[col].y += cf.y * sc;
@@ -322,9 +324,10 @@ static bool flat_ranks_exact(const sed_sector* sec) {
     return cat->binom[(size_t)(cat->n_site - 1) * cat->bstride + cat->n_filled + 1] != 0xffffffffu;
 }
 
-template <typename M, typename R, bool F, typename TE, int LPR, int B, int BLOCK, bool FLAT = false>
+template <typename M, typename R, bool F, typename TE, int LPR, int B, int BLOCK, bool FLAT = false, bool AMP = false>
 static int launch_shape(const MvConst& c, sed_sector* sec, const typename C2<R>::T* xh, typename C2<R>::T* y,
                         int chunk, hipStream_t s) {
+    using T = typename C2<R>::T;
     size_t off[9];
     const size_t lds = mv_lds_layout<R>(c, off, BLOCK, FLAT, !FLAT);
     if ((int)lds > sec->cat->ctx->max_lds) return fail(SED_EUNSUPPORTED, "launch shape needs too much LDS");
@@ -336,17 +339,32 @@ static int launch_shape(const MvConst& c, sed_sector* sec, const typename C2<R>:
     const TE* table;
     if constexpr (sizeof(TE) == 8) table = sec->d_table64.p;
     else table = sec->d_table.p;
-    hipLaunchKernelGGL((k_matvec<M, R, F, LPR, B, BLOCK, FLAT, TE>), dim3((unsigned)grid), dim3(BLOCK), lds, s, c,
-                       sec->d_rep.p, sec->d_stab.p, table, xh, y, nb, ch);
+    const T* amp = nullptr;
+    if constexpr (AMP) {
+        if (!sec->amp) return fail(SED_ESTATE, "hop amplitudes missing");
+        if constexpr (sizeof(R) == 8) amp = reinterpret_cast<const T*>(sec->amp->a64.p);
+        else amp = reinterpret_cast<const T*>(sec->amp->a32.p);
+    }
+    hipLaunchKernelGGL((k_matvec<M, R, F, LPR, B, BLOCK, FLAT, TE, AMP>), dim3((unsigned)grid), dim3(BLOCK), lds, s,
+                       c, sec->d_rep.p, sec->d_stab.p, table, xh, y, nb, ch, amp);
     SED_HIP(hipGetLastError());
     return SED_OK;
 }
 
-// 8-byte-table sectors (TE = uint64_t) run the site-by-site shapes only.
+// 8-byte-table sectors (TE = uint64_t) run the site-by-site shapes only; sectors with
+// per-rank hop amplitudes (site phases not of gauge form, 4-byte tables) run shape 1.
+constexpr int kAmpShape = 1;
 template <typename M, typename R, bool F, typename TE>
 static int run_shape(sed_sector* sec, int shape, int chunk, const typename C2<R>::T* xh, typename C2<R>::T* y,
                      hipStream_t s) {
     MvConst c = make_const<R>(sec);
+    if (sec->amp) {
+        if constexpr (sizeof(TE) == 4) {
+            if (shape == kAmpShape) return launch_shape<M, R, F, TE, 1, 4, 256, false, true>(c, sec, xh, y, chunk, s);
+        }
+        return fail(SED_EUNSUPPORTED, "launch shape " + std::to_string(shape) +
+                                          " with site phases not of gauge form (shape 1 only)");
+    }
     switch (shape) {
         case 0: return launch_shape<M, R, F, TE, 1, 1, 256>(c, sec, xh, y, chunk, s);
         case 1: return launch_shape<M, R, F, TE, 1, 4, 256>(c, sec, xh, y, chunk, s);
@@ -377,6 +395,7 @@ static bool shape_fits(const sed_sector* sec, int shape) {
     MvConst c = make_const<R>(sec);
     size_t off[9];
     if (kShapes[shape].flat && (sec->wide || !flat_ranks_exact(sec))) return false;
+    if (sec->amp && shape != kAmpShape) return false;
     return (int)mv_lds_layout<R>(c, off, kShapes[shape].block, kShapes[shape].flat, !kShapes[shape].flat) <=
            sec->cat->ctx->max_lds;
 }
@@ -637,12 +656,13 @@ int launch_dense(sed_sector* sec, double* H_dev, hipStream_t s) {
     const int64_t nb = (sec->dim + kMvBlock - 1) / kMvBlock;
     const bool F = sec->cat->stats == SED_FERMIONIC;
     auto H = reinterpret_cast<double2*>(H_dev);
+    const auto* A = sec->amp ? reinterpret_cast<const double2*>(sec->amp->a64.p) : nullptr;
     if (sec->cat->n_site <= 32) {
-        if (F) hipLaunchKernelGGL((k_dense<uint32_t, true>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, H);
-        else hipLaunchKernelGGL((k_dense<uint32_t, false>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, H);
+        if (F) hipLaunchKernelGGL((k_dense<uint32_t, true>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, A, H);
+        else hipLaunchKernelGGL((k_dense<uint32_t, false>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, A, H);
     } else {
-        if (F) hipLaunchKernelGGL((k_dense<uint64_t, true>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, H);
-        else hipLaunchKernelGGL((k_dense<uint64_t, false>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, H);
+        if (F) hipLaunchKernelGGL((k_dense<uint64_t, true>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, A, H);
+        else hipLaunchKernelGGL((k_dense<uint64_t, false>), dim3((unsigned)nb), dim3(kMvBlock), lds, s, c, sec->d_rep.p, sec->d_stab.p, sec->d_table.p, A, H);
     }
     SED_HIP(hipGetLastError());
     return SED_OK;

@@ -225,12 +225,14 @@ __device__ __forceinline__ void row_sums(const MvLds<R>& L, const MvConst& c, co
 // target's binomial row offset, and the hop's sign (table parity xor Jordan-Wigner parity)
 // selects the signed character chi2[g << 1 | sign]; hops that are gated out or leave the
 // sector select chi2[1 << shift] = 0.  Each hop then adds conj(t') chi2 xhat[c] with four
-// fused multiply-adds.
-template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST, typename TE>
+// fused multiply-adds.  With AMP (site phases not of gauge form) each hop also reads the
+// target rank's hop amplitude amp[rank], issued beside its table read, and adds
+// conj(t') chi2 amp xhat[c].
+template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST, typename TE, bool AMP = false>
 __device__ __forceinline__ void row_sums_padded(const MvLds<R>& L, const MvConst& c, const M m, const int sub,
                                                 uint32_t* S, const TE* __restrict__ table,
                                                 const typename C2<R>::T* __restrict__ xhat, R& ax, R& ay, R& dx,
-                                                R& dy) {
+                                                R& dy, const typename C2<R>::T* __restrict__ amp = nullptr) {
     using T = typename C2<R>::T;
     static_assert(kTermPad % BATCH == 0, "batch must divide the term padding");
     const uint32_t mask2 = (1u << c.shift) - 1u;
@@ -281,9 +283,11 @@ __device__ __forceinline__ void row_sums_padded(const MvLds<R>& L, const MvConst
             }
             constexpr TE none = ~(TE)0;
             TE e[BATCH];
+            T am[BATCH];
 #pragma unroll
             for (int u = 0; u < BATCH; ++u) {
                 const TE v = table[ok[u] ? idx[u] : 0u];
+                if constexpr (AMP) am[u] = amp[ok[u] ? idx[u] : 0u];
                 e[u] = ok[u] ? v : none;
             }
             T v[BATCH];
@@ -292,7 +296,8 @@ __device__ __forceinline__ void row_sums_padded(const MvLds<R>& L, const MvConst
 #pragma unroll
             for (int u = 0; u < BATCH; ++u) {
                 const uint32_t ci = e[u] != none ? ((uint32_t)e[u] ^ pj[u]) & mask2 : dead;
-                const T cf = cmul(L.tcoef[t + u], L.chi[ci]);
+                T cf = cmul(L.tcoef[t + u], L.chi[ci]);
+                if constexpr (AMP) cf = cmul(cf, am[u]);
                 ax += cf.x * v[u].x;
                 ax -= cf.y * v[u].y;
                 ay += cf.x * v[u].y;

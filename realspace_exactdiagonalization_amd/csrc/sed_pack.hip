@@ -396,7 +396,7 @@ static size_t pack_lds(const sed_sector* sec) {
 }
 
 bool pack_eligible(const sed_sector* sec, int nev) {
-    if (sec->wide || sec->dim <= 500 || sec->dim > kPackMaxDim || nev > kPackMaxNev || nev > krylov_dim(sec->dim, nev) - 2)
+    if (sec->wide || sec->amp || sec->dim <= 500 || sec->dim > kPackMaxDim || nev > kPackMaxNev || nev > krylov_dim(sec->dim, nev) - 2)
         return false;
     // static LDS of the kernel is below 4 KiB
     return pack_lds(sec) + 4096 <= (size_t)sec->cat->ctx->max_lds;

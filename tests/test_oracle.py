@@ -228,6 +228,73 @@ def test_union_of_sector_spectra_is_full_spectrum(oracle_mod, case):
     assert np.abs(np.sort(allv) - full).max() < 1e-10
 
 
+def fock_group_operator(model, group, g):
+    """U_g|s> = alpha_g(s)|g(s)> on all C(N, k) states, from the documented convention of
+    apply_operation_to_mask (src/symmetry_resolved_ed.jl:124-131): alpha = product of the
+    site phases of the occupied sites times, for fermions, (-1)^(number of inversions of
+    the placement), counted here pair by pair."""
+    N, k = model.n_site, model.n_filled
+    states = list(itertools.combinations(range(N), k))
+    index = {s: i for i, s in enumerate(states)}
+    ferm = model.statistics == L.FERMIONIC
+    U = np.zeros((len(states), len(states)), dtype=np.complex128)
+    for j, s in enumerate(states):
+        targets = [int(group.perm[g, i]) - 1 for i in s]
+        amp = 1 + 0j
+        for i in s:
+            amp *= group.phases[g, i]
+        inv = sum(1 for a in range(k) for b in range(a + 1, k) if targets[a] > targets[b])
+        if ferm and inv % 2:
+            amp = -amp
+        U[index[tuple(sorted(targets))], j] = amp
+    return U
+
+
+@pytest.mark.parametrize("stats", [L.BOSONIC, L.FERMIONIC])
+@pytest.mark.parametrize("m", [1, 2])
+def test_general_site_phases_union_of_sectors(oracle_mod, stats, m):
+    """Site phases that are not of gauge form (the wheel: a phase on the centre, which every
+    rotation fixes): the brute-force operators form a representation that commutes with H,
+    some stabilizer phase is complex (so no sign-only table could hold it), every oracle
+    sector matrix is Hermitian and the union of the sector spectra is the full spectrum."""
+    from tests.wheel_model import wheel
+    model, G, irreps = wheel(6, m, 4, stats)
+    H = full_fock_hamiltonian(model)
+    assert np.abs(H - H.conj().T).max() < 1e-13
+    U = [fock_group_operator(model, G, g) for g in range(G.order)]
+    for g in range(G.order):
+        assert np.abs(H @ U[g] - U[g] @ H).max() < 1e-12
+        for h in range(G.order):
+            comp = G.perm[g][G.perm[h] - 1]
+            kk = next(c for c in range(G.order) if np.array_equal(G.perm[c], comp))
+            assert np.abs(U[g] @ U[h] - U[kk]).max() < 1e-12
+    cat = oracle_mod.Catalog(model.n_site, model.n_filled, G, stats == L.FERMIONIC)
+    _, _, gi, gp = cat.orbits()
+    assert np.abs(gp[gi > 0].imag).max() > 0.5
+    full = np.linalg.eigvalsh(H)
+    allv = []
+    for ir in irreps:
+        sec = oracle_mod.Sector(cat, ir.values)
+        if sec.dim:
+            sec.set_terms(*L.term_arrays(model))
+            Hs = sec.dense()
+            assert np.abs(Hs - Hs.conj().T).max() < 1e-12
+            allv.extend(np.linalg.eigvalsh(Hs))
+    assert len(allv) == len(full)
+    assert np.abs(np.sort(allv) - full).max() < 1e-10
+    # a planted mistake: dropping the centre phase leaves the same orbits but the wrong
+    # sectors (their union no longer reproduces the spectrum)
+    G0 = L.SymmetryGroup(G.name, G.perm, np.ones_like(G.phases), G.labels, G.identity_idx)
+    cat0 = oracle_mod.Catalog(model.n_site, model.n_filled, G0, stats == L.FERMIONIC)
+    wrong = []
+    for ir in irreps:
+        sec = oracle_mod.Sector(cat0, ir.values)
+        if sec.dim:
+            sec.set_terms(*L.term_arrays(model))
+            wrong.extend(np.linalg.eigvalsh(0.5 * (sec.dense() + sec.dense().conj().T)))
+    assert len(wrong) != len(full) or np.abs(np.sort(wrong) - full).max() > 1e-3
+
+
 def test_twist_periodicity(oracle_mod):
     """I2 (test/flux_regression.jl:59): the global spectrum at twist 0 equals the one at
     twist 1.  The flux-aware group at twist 1 relabels momenta (each particle gains
