@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(BLOCK) k_matvec(MvConst c, const uint64_t* __r
 // 64 / lanes), hops per batch and workgroup size.  A shape whose LDS (rank scratch of
 // k entries per thread) does not fit a workgroup is skipped.
 // Shapes with `flat` walk each lane's hops in batches across site boundaries
-// (row_sums_flat), the others site by site (row_sums).
+// (row_sums_flat), the others site by site (row_sums_padded).
 struct MvShape {
     int lanes, batch, block;
     bool flat;

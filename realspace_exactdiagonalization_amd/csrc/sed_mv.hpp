@@ -52,7 +52,7 @@ struct MvLds {
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-// Rank scratch entries per thread: k (per-site S of row_sums) or 2 (k + 1) (the row's
+// Rank scratch entries per thread: k (per-site S of the site-by-site loops) or 2 (k + 1) (the row's
 // prefix arrays of row_sums_flat).
 __host__ __device__ inline int mv_scratch(const MvConst& c, bool flat) {
     return flat ? 2 * (c.k + 1) : (c.k > 0 ? c.k : 1);
@@ -141,86 +141,8 @@ __device__ __forceinline__ MvLds<R> stage_mv(const MvConst& c, int block, bool f
 //     S[j] = sum_{i<j} (C(q_i, i+1) - C(q_i, i+2))    (uint32, modulo 2^32),
 // one pass over mp's bits per site.  Offsets into the binomial table are 24-bit
 // products (site < 64, stride < 70).
-template <typename M, typename R, bool FERMI, int LPR, int BATCH, int SST, typename TE>
-__device__ __forceinline__ void row_sums(const MvLds<R>& L, const MvConst& c, const M m, const int sub,
-                                         uint32_t* S, const TE* __restrict__ table,
-                                         const typename C2<R>::T* __restrict__ xhat, R& ax, R& ay, R& dx,
-                                         R& dy) {
-    using T = typename C2<R>::T;
-    M rest = m;
-    for (int i = 0; i < sub && rest; ++i) rest &= rest - 1;
-    int ia = sub;
-    while (rest) {
-        const int a = ctz_m<M>(rest);
-        for (int i = 0; i < LPR && rest; ++i) rest &= rest - 1;
-        const M mp = m ^ ((M)1 << a);
-        uint32_t base = 0;
-        {
-            uint32_t cur = 0;
-            S[0] = 0;
-            M t = mp;
-            int j = 0;
-            while (t) {
-                const int q = ctz_m<M>(t);
-                t &= t - 1;
-                const uint32_t* bq = L.bin + __umul24(q, c.bs) + j;
-                const uint32_t c1 = bq[1], c2 = bq[2];
-                base += c2;
-                cur += c1 - c2;
-                ++j;
-                S[j * SST] = cur;
-            }
-        }
-        for (int d = L.dstart[a]; d < L.dstart[a + 1]; ++d) {
-            if ((m >> L.dj[d]) & 1) {
-                dx += L.dv[d].x;
-                dy += L.dv[d].y;
-            }
-        }
-        // BATCH hops at a time, without branches: every candidate computes its rank; the
-        // gated-out ones (target occupied, or past the site's list) read entry 0 of the
-        // table and of xhat (valid addresses, cache-resident) and contribute zero, so the
-        // BATCH table reads, then the BATCH gathers, are issued back to back.
-        const int t0 = L.tstart[a], t1 = L.tstart[a + 1];
-        for (int t = t0; t < t1; t += BATCH) {
-            uint32_t idx[BATCH];
-            bool ok[BATCH];
-            int pj[BATCH];
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
-                const int tt = min(t + u, t1 - 1);
-                const int b = L.tto[tt];
-                const int ib = popc_m<M>(mp & (((M)1 << b) - 1));
-                ok[u] = (t + u < t1) && !((m >> b) & 1);
-                idx[u] = base + S[ib * SST] + L.bin[__umul24(b, c.bs) + ib + 1];
-                pj[u] = FERMI ? ((ib - ia) & 1) : 0;
-            }
-            constexpr TE none = ~(TE)0;
-            TE e[BATCH];
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
-                const TE v = table[ok[u] ? idx[u] : 0u];
-                e[u] = ok[u] ? v : none;
-            }
-            T v[BATCH];
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) v[u] = xhat[e[u] != none ? (e[u] >> c.shift) : (TE)0];
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
-                const bool live = e[u] != none;
-                const int g = live ? (int)(((uint32_t)e[u] >> 1) & c.gmask) : 0;
-                const T cf = cmul(L.tcoef[min(t + u, t1 - 1)], L.chi[g]);
-                const T pr = cmul(cf, v[u]);
-                const R sg = live ? (((((uint32_t)e[u] & 1) ^ (uint32_t)pj[u]) != 0) ? (R)-1 : (R)1) : (R)0;
-                ax += sg * pr.x;
-                ay += sg * pr.y;
-            }
-        }
-        ia += LPR;
-    }
-}
-
-// row_sums over the padded term lists (stage_mv with padded = true): every site's list is
+//
+// The term lists are the padded ones (stage_mv with padded = true): every site's list is
 // a multiple of kTermPad long, so a batch needs no bounds test, each entry carries its
 // target's binomial row offset, and the hop's sign (table parity xor Jordan-Wigner parity)
 // selects the signed character chi2[g << 1 | sign]; hops that are gated out or leave the

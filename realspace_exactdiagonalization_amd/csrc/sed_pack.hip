@@ -41,7 +41,8 @@ constexpr int64_t kPackMaxDim = 1 << 15;   // one CU per sector: larger sectors 
 constexpr int kPackMaxNev = kProMaxNev;
 constexpr int kPackDots = 4;              // basis vectors per sweep of the dot products
 constexpr int kPackBatch = 4;             // hops whose table reads (then gathers) issue together
-                                          // (flux scan: 2 -> 1097, 4 -> 1162, 8 -> 912 solves/s)
+                                          // (divides the term padding; flux scan, FP64 batched:
+                                          // 1287-1305 solves/s, 1225 with the unpadded loop)
 constexpr int kMoveRows = 64;             // rows per workgroup of the restart kernel
 constexpr int kSlotDoubles = 2 * kMaxBasis + 2;   // alpha[m], beta[m], full steps, (int step, int flag)
 
@@ -182,7 +183,7 @@ __global__ void __launch_bounds__(kPackBlock) k_pack_lanczos(const PackItem* __r
     __shared__ int sh_lo, sh_need;
     const PackItem& it = items[blockIdx.x];
     ProState* pro = it.pro;
-    const MvLds<R> L = stage_mv<R>(it.c, NT);
+    const MvLds<R> L = stage_mv<R>(it.c, NT, false, true);
     T* V = reinterpret_cast<T*>(it.V);
     T* w = reinterpret_cast<T*>(it.w);
     T* xh = reinterpret_cast<T*>(it.xhat);
@@ -240,7 +241,8 @@ __global__ void __launch_bounds__(kPackBlock) k_pack_lanczos(const PackItem* __r
         for (int64_t r = threadIdx.x; r < n; r += NT) {
             const M mm = (M)it.rep[r];
             R ax = 0, ay = 0, dx = 0, dy = 0;
-            row_sums<M, R, FERMI, 1, kPackBatch, NT>(L, it.c, mm, 0, L.S + threadIdx.x, it.table, xh, ax, ay, dx, dy);
+            row_sums_padded<M, R, FERMI, 1, kPackBatch, NT, uint32_t>(L, it.c, mm, 0, L.S + threadIdx.x, it.table, xh,
+                                                                     ax, ay, dx, dy);
             const R inv = (R)1 / sqrt((R)it.stab[r]);
             const T xr = xh[r];
             T out;
@@ -391,8 +393,8 @@ class Arena {
 // LDS of the packed kernel for this sector (dynamic part: term tables + rank scratch)
 static size_t pack_lds(const sed_sector* sec) {
     size_t off[9];
-    return std::max(mv_lds_layout<double>(mv_const(sec, true), off, kPackBlock),
-                    mv_lds_layout<float>(mv_const(sec, false), off, kPackBlock));
+    return std::max(mv_lds_layout<double>(mv_const(sec, true), off, kPackBlock, false, true),
+                    mv_lds_layout<float>(mv_const(sec, false), off, kPackBlock, false, true));
 }
 
 bool pack_eligible(const sed_sector* sec, int nev) {
@@ -761,7 +763,7 @@ class PackEngine {
             q.start = kStartNone;
             q.set_step = -1;
             size_t off[9];
-            lds = std::max(lds, mv_lds_layout<R>(it.c, off, kPackBlock));
+            lds = std::max(lds, mv_lds_layout<R>(it.c, off, kPackBlock, false, true));
         }
         PackItem* d_items = reinterpret_cast<PackItem*>(items_.p) + offset;
         SED_HIP(hipMemcpyAsync(d_items, h_items_ + offset, idx.size() * sizeof(PackItem), hipMemcpyHostToDevice, st_));
