@@ -17,13 +17,14 @@ value = roofline.achieved = algorithmic bytes of the kernel in this library's da
 layout / its mean duration.  Algorithmic bytes per matvec (s = 8 for c32, 16 for c64;
 D, M_g, M counted exactly on the device):
   D * (8 rep mask + 1 stabilizer order + s x[row] + s y[row])
-  + M_g * 4   (one 4-byte sector-table entry per gated hop)
+  + M_g * 4   (one 4-byte sector-table entry per gated hop; 8 bytes on sectors whose
+               rows do not fit the 4-byte entry, e.g. --workload xxz34)
   + M   * s   (one gather of xhat per in-sector hop)
 The workload's own per-unit figures (an 8-byte table entry and a 4-byte norm per row)
 give workload_bytes_per_launch, reported beside it; they count bytes this layout does not
 move, so their rate is not an HBM fraction.  roofline.traffic is the HBM traffic per
 launch measured by rocprofv3 counters (FETCH_SIZE and WRITE_SIZE, copy-calibrated) in
-the committed profile named in roofline.traffic_source.
+the committed profile named in roofline.traffic_source (null for other workloads).
 """
 import argparse
 import json
@@ -50,21 +51,30 @@ def algorithmic_bytes(dim, gated, ins, s):
     return dim * (8 + 4 + 2 * s) + gated * 8 + ins * s
 
 
-def layout_bytes(dim, gated, ins, s):
+def table_entry_bytes(n_site, group_order, dim):
+    """Bytes of one sector-table entry: row << shift | g << 1 | sign in 4 bytes, or 8 when
+    the row does not fit beside shift = ceil(log2 |G|) + 1 bits (sectors above 32 sites)."""
+    shift = max(0, (group_order - 1).bit_length()) + 1
+    return 8 if n_site > 32 and dim > (1 << (32 - shift)) - 1 else 4
+
+
+def layout_bytes(dim, gated, ins, s, entry=4):
     """The same count for this library's compressed layout: 1-byte stabilizer order and
-    a 4-byte table entry (row, group element and sign packed)."""
-    return dim * (8 + 1 + 2 * s) + gated * 4 + ins * s
+    a 4-byte (8-byte on the largest sectors) table entry (row, group element and sign packed)."""
+    return dim * (8 + 1 + 2 * s) + gated * entry + ins * s
 
 
 COUNTER_PROFILE = os.path.join("profiles", "matvec_checker4x6_counters.md")
+COUNTER_WORKLOAD = "checker4x6"
 
 
-def measured_traffic(precision):
+def measured_traffic(precision, workload):
     """(HBM bytes per launch of the sector matvec on this workload, source) from the
     committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, scaled by the
-    plain-copy calibration); (None, None) when the summary has no line for it."""
+    plain-copy calibration); (None, None) when the summary has no line for it or covers
+    another workload."""
     path = os.path.join(ROOT, COUNTER_PROFILE)
-    if not os.path.exists(path):
+    if workload != COUNTER_WORKLOAD or not os.path.exists(path):
         return None, None
     key = f"traffic_bytes_{precision}:"
     src = None
@@ -529,9 +539,9 @@ def main():
         w0 = time.time()
         ms = H.benchmark(prec, args.steps)
         wall = time.time() - w0
-        B = layout_bytes(dim, gated, ins, s)
+        B = layout_bytes(dim, gated, ins, s, table_entry_bytes(model.n_site, G.order, dim))
         Bw = algorithmic_bytes(dim, gated, ins, s)
-        traffic, src = measured_traffic(name)
+        traffic, src = measured_traffic(name, args.workload)
         res[name] = {"ms": ms, "bytes": B, "gbs": B / (ms * 1e-3) / 1e9, "wall_s": wall,
                      "workload_bytes": Bw, "workload_gbs": Bw / (ms * 1e-3) / 1e9, "shape": basis.shape(prec),
                      "traffic": traffic, "traffic_source": src}
