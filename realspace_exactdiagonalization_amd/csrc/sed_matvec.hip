@@ -351,8 +351,8 @@ static int launch_shape(const MvConst& c, sed_sector* sec, const typename C2<R>:
     return SED_OK;
 }
 
-// 8-byte-table sectors (TE = uint64_t) run the site-by-site shapes only; sectors with
-// per-rank hop amplitudes (site phases not of gauge form, 4-byte tables) run shape 1.
+// Sectors with per-rank hop amplitudes (site phases not of gauge form, 4-byte tables) run
+// shape 1 only.
 constexpr int kAmpShape = 1;
 template <typename M, typename R, bool F, typename TE>
 static int run_shape(sed_sector* sec, int shape, int chunk, const typename C2<R>::T* xh, typename C2<R>::T* y,
@@ -375,17 +375,10 @@ static int run_shape(sed_sector* sec, int shape, int chunk, const typename C2<R>
         case 6: return launch_shape<M, R, F, TE, 2, 4, 128>(c, sec, xh, y, chunk, s);
         case 7: return launch_shape<M, R, F, TE, 1, 4, 512>(c, sec, xh, y, chunk, s);
         case 8: return launch_shape<M, R, F, TE, 2, 4, 512>(c, sec, xh, y, chunk, s);
-        default: break;
-    }
-    if constexpr (sizeof(TE) == 8) {
-        return fail(SED_EUNSUPPORTED, "flat hop loop on an 8-byte sector table");
-    } else {
-        switch (shape) {
-            case 9: return launch_shape<M, R, F, TE, 1, 4, 256, true>(c, sec, xh, y, chunk, s);
-            case 10: return launch_shape<M, R, F, TE, 1, 8, 256, true>(c, sec, xh, y, chunk, s);
-            case 11: return launch_shape<M, R, F, TE, 2, 4, 256, true>(c, sec, xh, y, chunk, s);
-            default: return launch_shape<M, R, F, TE, 2, 8, 256, true>(c, sec, xh, y, chunk, s);
-        }
+        case 9: return launch_shape<M, R, F, TE, 1, 4, 256, true>(c, sec, xh, y, chunk, s);
+        case 10: return launch_shape<M, R, F, TE, 1, 8, 256, true>(c, sec, xh, y, chunk, s);
+        case 11: return launch_shape<M, R, F, TE, 2, 4, 256, true>(c, sec, xh, y, chunk, s);
+        default: return launch_shape<M, R, F, TE, 2, 8, 256, true>(c, sec, xh, y, chunk, s);
     }
 }
 
@@ -394,7 +387,7 @@ template <typename R>
 static bool shape_fits(const sed_sector* sec, int shape) {
     MvConst c = make_const<R>(sec);
     size_t off[9];
-    if (kShapes[shape].flat && (sec->wide || !flat_ranks_exact(sec))) return false;
+    if (kShapes[shape].flat && !flat_ranks_exact(sec)) return false;
     if (sec->amp && shape != kAmpShape) return false;
     return (int)mv_lds_layout<R>(c, off, kShapes[shape].block, kShapes[shape].flat, !kShapes[shape].flat) <=
            sec->cat->ctx->max_lds;

@@ -197,8 +197,8 @@ def test_full_size_eigenpairs_satisfy_the_oracle_rows(ed, oracle_mod, full, name
 def test_xxz34_every_launch_shape_on_the_8_byte_table(ed, oracle_mod, full):
     """The k = 0 sector of XXZ N=34 (68,635,478 rows at |G| = 34) does not fit the 4-byte
     table entry (row < 2^25 beside 7 group and parity bits) and runs on the 8-byte table:
-    every site-by-site launch shape against the oracle's rows, FP64 and FP32; the flat hop
-    loop is refused with SED_EUNSUPPORTED."""
+    every launch shape (site by site and flat: C(33, 18) < 2^32) against the oracle's rows,
+    FP64 and FP32."""
     fs = full("xxz34", (0.0, 0.0))
     assert fs.b.dim == 68635478 and fs.b.dim >= 1 << 25
     assert fs.check_basis(oracle_mod) == ((0, -1), True)
@@ -225,7 +225,7 @@ def test_xxz34_every_launch_shape_on_the_8_byte_table(ed, oracle_mod, full):
             ran += 1
         fs.b.set_shape(prec, -1)
     fs.H.precision = ed.PREC_FP64
-    assert ran == 2 * 9 and refused == 2 * (n_shapes - 9)
+    assert ran == 2 * n_shapes and refused == 0
 
 
 def catalog(ed, ctx, m, G):
