@@ -25,6 +25,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 
 #include "sed_device.hpp"
@@ -488,13 +489,29 @@ static bool graphs_enabled() {
     return !(e && e[0] == '0');
 }
 
+// SED_SOLVE_EVENTS=1: every Lanczos loop launches its steps eagerly with HIP events between
+// the parts of a step and prints, at its end, the GPU time of each part summed over the
+// steps (H|psi>, first Gram-Schmidt pass with its check, gated second pass, finalisation and
+// scaling, gaps between consecutive steps), and the wall time of the thick restarts and of
+// the host checks, to stderr.  Read once per loop.
+static bool solve_events_enabled() {
+    const char* e = std::getenv("SED_SOLVE_EVENTS");
+    return e && e[0] == '1';
+}
+
 // Device state block (doubles): h1 (2*(m+2)), h2 (2*(m+2)), alpha (m), beta (m), scale (1);
 // ints after it: step, flag, gate; then the partial re-orthogonalisation state.
 template <typename T>
 class ThickLanczos {
    public:
     ThickLanczos(sed_sector* sec, int m, Workspace& ws)
-        : sec_(sec), n_(sec->dim), m_(m), ws_(ws), st_(ws.stream), mfma_(reorth_mfma(sec->dim)), pro_on_(pro_enabled()) {}
+        : sec_(sec), n_(sec->dim), m_(m), ws_(ws), st_(ws.stream), mfma_(reorth_mfma(sec->dim)), pro_on_(pro_enabled()),
+          ev_on_(solve_events_enabled()) {}
+    ~ThickLanczos() {
+        for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+    }
+    ThickLanczos(const ThickLanczos&) = delete;
+    ThickLanczos& operator=(const ThickLanczos&) = delete;
 
     int init() {
         SED_TRY(VecTraits<T>::reserve(ws_, n_, m_));
@@ -559,7 +576,9 @@ class ThickLanczos {
         int jm = 0;
         bool done = false, breakdown = false;
         // first matvec eagerly: it fixes the launch shape (autotuner) before any capture
-        const bool use_graph = graphs_enabled();
+        const bool use_graph = graphs_enabled() && !ev_on_;
+        if (ev_on_) SED_TRY(events_init());
+        const auto t_run = std::chrono::steady_clock::now();
         while (!done) {
             int j = k;
             while (j < m_) {
@@ -573,6 +592,8 @@ class ThickLanczos {
                 SED_HIP(hipMemcpyAsync(ws_.hcoef, alpha_, (2 * na + 1) * sizeof(double), hipMemcpyDeviceToHost, st_));
                 SED_HIP(hipMemcpyAsync(ws_.hcoef + 2 * na + 2, step_, 2 * sizeof(int), hipMemcpyDeviceToHost, st_));
                 SED_HIP(hipStreamSynchronize(st_));
+                const auto t_check = std::chrono::steady_clock::now();
+                if (ev_on_) SED_TRY(events_collect());
                 const double* ha = ws_.hcoef;
                 const double* hb = ws_.hcoef + na;
                 int flag = 0;
@@ -598,11 +619,13 @@ class ThickLanczos {
                     const double res = beta * std::fabs(last[i]);
                     if (res <= tol) ++conv; else break;
                 }
+                if (ev_on_) ev_host_ms_ += ms_since(t_check);
                 if (breakdown || conv >= nev || (j == m_ && out.restarts >= maxiter)) {
                     done = true;
                     break;
                 }
             }
+            const auto t_restart = std::chrono::steady_clock::now();
             eig(jm, T_, theta, Sm);
             if (done) break;
             // thick restart: kk Ritz vectors + the residual direction v_m
@@ -625,6 +648,7 @@ class ThickLanczos {
             SED_TRY(set_step_and_xhat(kk));
             k = kk;
             ++out.restarts;
+            if (ev_on_) ev_restart_ms_ += ms_since(t_restart);
         }
         const int nk = std::min(nev, jm);
         out.evals.assign(theta.begin(), theta.begin() + nk);
@@ -640,6 +664,17 @@ class ThickLanczos {
         SED_TRY(ritz(jm, Sk, nk));
         nritz_ = nk;
         SED_TRY(read_full_steps(out.full_steps));
+        if (ev_on_) {
+            const double tot = ms_since(t_run);
+            std::fprintf(stderr,
+                         "[sed-solve-events] %s dim=%lld m=%d steps=%d restarts=%d full_orth_steps=%d wall_ms=%.3f "
+                         "matvec_ms=%.3f pass1_ms=%.3f pass2_ms=%.3f finalize_scale_ms=%.3f step_gaps_ms=%.3f "
+                         "restart_wall_ms=%.3f host_check_wall_ms=%.3f\n",
+                         VecTraits<T>::fp64 ? "fp64" : "fp32", (long long)n_, m_, out.iters, out.restarts,
+                         out.full_steps, tot, ev_ms_[0], ev_ms_[1], ev_ms_[2], ev_ms_[3], ev_ms_[4], ev_restart_ms_,
+                         ev_host_ms_);
+            std::fflush(stderr);
+        }
         return SED_OK;
     }
 
@@ -673,20 +708,58 @@ class ThickLanczos {
 
     // one Lanczos step as plain launches
     int launch_step() {
+        hipEvent_t* ev = nullptr;   // 5 events of this step when timing, else none
+        if (ev_on_ && (ev_n_ + 1) * kEvPerStep <= (int)ev_.size()) ev = &ev_[(size_t)(ev_n_++) * kEvPerStep];
+        if (ev) SED_HIP(hipEventRecord(ev[0], st_));
         SED_TRY(launch_matvec_xhat(sec_, xh_, w_, VecTraits<T>::fp64, st_));
+        if (ev) SED_HIP(hipEventRecord(ev[1], st_));
         // first classical Gram-Schmidt pass over v_lo..v_j (lo decided on the device) into
         // v_{j+1}; a second pass over the whole basis, in place, only when the device
         // decides so (partial re-orthogonalisation or cancellation)
         SED_TRY(launch_orth_pass(h1_, &pro_->lo, 0, nullptr));
         hipLaunchKernelGGL(k_reorth_check, dim3(1), dim3(kRedBlock), 0, st_, step_, normpart_, kUpdGrid, h1_, h2_,
                            gate_, pro_, alpha_, beta_);
+        if (ev) SED_HIP(hipEventRecord(ev[2], st_));
         SED_TRY(launch_orth_pass(h2_, nullptr, 1, gate_));
+        if (ev) SED_HIP(hipEventRecord(ev[3], st_));
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kRedBlock), 0, st_, step_, h1_, h2_, normpart_, kUpdGrid, alpha_,
                            beta_, scale_, flag_, pro_);
         hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n_)), dim3(256), 0, st_, V_, (int64_t)n_, step_, scale_,
                            sec_->d_stab.p, xh_);
+        if (ev) SED_HIP(hipEventRecord(ev[4], st_));
         SED_HIP(hipGetLastError());
         return SED_OK;
+    }
+
+    // event timing of the steps (SED_SOLVE_EVENTS=1)
+    static constexpr int kEvPerStep = 5;
+    static constexpr int kEvSteps = 64;   // more than the steps between two host checks
+    int events_init() {
+        if (!ev_.empty()) return SED_OK;
+        ev_.resize((size_t)kEvSteps * kEvPerStep, nullptr);
+        for (hipEvent_t& e : ev_) SED_HIP(hipEventCreate(&e));
+        return SED_OK;
+    }
+    // after a stream synchronisation: add the recorded steps' part times, then reuse the pool
+    int events_collect() {
+        for (int s = 0; s < ev_n_; ++s) {
+            const hipEvent_t* e = &ev_[(size_t)s * kEvPerStep];
+            for (int p = 0; p < 4; ++p) {
+                float ms = 0.f;
+                SED_HIP(hipEventElapsedTime(&ms, e[p], e[p + 1]));
+                ev_ms_[p] += ms;
+            }
+            if (s + 1 < ev_n_) {
+                float ms = 0.f;
+                SED_HIP(hipEventElapsedTime(&ms, e[4], e[kEvPerStep]));
+                ev_ms_[4] += ms;
+            }
+        }
+        ev_n_ = 0;
+        return SED_OK;
+    }
+    static double ms_since(std::chrono::steady_clock::time_point t0) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 
     // One Gram-Schmidt pass: h = V[lo..j]^H w (and |w|^2), then w - V h into v_{j+1},
@@ -830,6 +903,11 @@ class ThickLanczos {
     ProState* pro_ = nullptr;
     bool mfma_ = true;
     bool pro_on_ = true;
+    bool ev_on_ = false;
+    std::vector<hipEvent_t> ev_;
+    int ev_n_ = 0;
+    double ev_ms_[5] = {0, 0, 0, 0, 0};
+    double ev_restart_ms_ = 0, ev_host_ms_ = 0;
 };
 
 int64_t solve_footprint(const sed_sector* sec, int nev, int precision) {
