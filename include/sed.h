@@ -82,8 +82,9 @@ int sed_destroy(sed_context* ctx);
  * sign-only tables.  Other site phases are accepted when the group's operators form a
  * representation (the permutations are closed under composition and
  * eta_{g o h}(i) = eta_h(i) eta_g(perm_h(i)), e.g. a phase on a site that rotations
- * fix): the catalog then also keeps a per-rank hop amplitude (24 B per rank) and H|psi>
- * runs one launch shape; phases that are neither give SED_EUNSUPPORTED.  Requires n_site <= 64,
+ * fix): the catalog then also keeps a per-rank hop amplitude (24 B per rank, catalogs of
+ * at most 2^31 - 1 ranks) and H|psi> runs one launch shape; phases that are neither give
+ * SED_EUNSUPPORTED.  Requires n_site <= 64,
  * C(n_site, n_filled) <= 2^32 - 2 (colex ranks are 32-bit), at most 2^31 orbits and
  * n_group <= 64.  Sectors built on it hold at most 2^31 - 1 rows. */
 int sed_catalog_build(sed_context* ctx, int n_site, int n_filled, int statistics,

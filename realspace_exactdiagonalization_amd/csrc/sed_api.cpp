@@ -271,6 +271,11 @@ int sed_catalog_build(sed_context* ctx, int n_site, int n_filled, int statistics
                 cat->binom[(size_t)a * cat->bstride + b] = (uint32_t)std::min<uint64_t>(C[a][b], 0xffffffffull);
     }
     cat->n_total = (int64_t)tot;
+    // per-rank hop amplitudes (24 B per rank) are refused where every sector of typical size
+    // would need the 8-byte table, which they do not run on
+    if (cat->general && cat->n_total >= ((int64_t)1 << 31))
+        return fail(SED_EUNSUPPORTED, "site phases not of gauge form on a catalog of " + std::to_string(tot) +
+                                          " ranks (at most 2^31 - 1)");
     hipStream_t s = ctx->stream;
     std::vector<uint8_t> p8((size_t)n_group * n_site), inv8(n_group);
     for (size_t i = 0; i < p8.size(); ++i) p8[i] = (uint8_t)(cat->perm[i] - 1);
@@ -386,6 +391,8 @@ int sed_catalog_update_phases(sed_catalog* cat, const double* phases) {
     int rc = set_group_phases(cat, phases);
     if (rc == SED_OK && cat->inv != old_inv)
         rc = fail(SED_EUNSUPPORTED, "the new phases change the inverse group elements (rebuild the catalog)");
+    if (rc == SED_OK && cat->general && cat->n_total >= ((int64_t)1 << 31))
+        rc = fail(SED_EUNSUPPORTED, "site phases not of gauge form on a catalog of more than 2^31 - 1 ranks");
     std::shared_ptr<HopAmp> amp;   // sectors built before keep the table of their phases
     if (rc == SED_OK && cat->general) {
         amp = std::make_shared<HopAmp>();

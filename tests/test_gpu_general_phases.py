@@ -121,7 +121,8 @@ def test_launch_shapes_and_refusals(ed, ctx, oracle_mod):
     """A sector of the wheel runs one launch shape (shape 1; the others are refused with
     SED_EUNSUPPORTED and the autotuner skips them); site phases that are not a
     representation (one rotation's phase on a ring site changed, its inverse's matching
-    phase with it) are refused at catalog build."""
+    phase with it), or that would need hop amplitudes for 2^31 ranks or more, are refused at
+    catalog build."""
     model, G, irreps = make("wheel16_m5_f")
     gc, oc = catalogs(ed, ctx, oracle_mod, model, G)
     gb = ed.build_symmetry_sector_basis(gc, irreps[3])
@@ -158,6 +159,12 @@ def test_launch_shapes_and_refusals(ed, ctx, oracle_mod):
         ed.build_symmetry_orbit_catalog(n_site=model.n_site, n_filled=model.n_filled, symmetry_group=bad,
                                         particle_statistics=model.statistics, ctx=ctx)
     assert e.value.code == _lib.SED_EUNSUPPORTED and "representation" in str(e.value)
+    # hop amplitudes are not kept for catalogs of 2^31 ranks or more (C(34, 17) here)
+    big, Gb, _ = wheel(33, 1, 17, L.BOSONIC)
+    with pytest.raises(_lib.SedError) as e:
+        ed.build_symmetry_orbit_catalog(n_site=big.n_site, n_filled=big.n_filled, symmetry_group=Gb,
+                                        particle_statistics=big.statistics, ctx=ctx)
+    assert e.value.code == _lib.SED_EUNSUPPORTED and "2^31" in str(e.value)
     gc.close()
 
 
