@@ -21,6 +21,32 @@
  *   build_matrixfree_projection_table :680-728
  *   _apply_table!                     :731-765 (per-thread buffers, static split)
  *   apply_hamiltonian!                :813-878 (on-the-fly, threaded over columns)
+ *   update_orbit_stabilizer_phases!   :338-369
+ *   _build_density_correlation_matrix! observables/static_structure_factor.jl:41-104
+ *   off_diagonal_long_range_order     observables/off_diagonal_long_range_order.jl:74-126
+ *   _expand_sector_state_to_fock_amplitudes observables/entanglement_spectrum.jl:46-79
+ *   _position_operator_matrix         observables/charge_pump.jl:47-91
+ * and two checkers for sectors too large for its catalog: a sector basis against its
+ * definition (oc_check_sector_basis) and sampled rows of H x without any table
+ * (oc_sample_rows, the projection-table formula of :718 evaluated per row).
+ *
+ * How each is pinned (tests/test_oracle.py, tests/test_checkpoint.py, tests/test_topology.py):
+ *   ranks, Gosper order, LinTable      the design notebook's cells 22/24; Gosper's sequence
+ *                                      visits ranks 0, 1, 2, ... over whole C(n, k) spaces
+ *   catalog tables, sector bases       the benchmark CSV's orbit counts and sector dims,
+ *                                      notebook cell 26, the character formula for every
+ *                                      momentum, the sectors partitioning C(N, k)
+ *   matrix elements, both matvecs      the CSV energies (36 rows), notebook cell 43 (nnz 756),
+ *                                      free-fermion closed forms (XX chain, Hubbard U=0, twisted),
+ *                                      brute-force Fock-space spectra (union of sectors =
+ *                                      full spectrum, also for site phases not of gauge form),
+ *                                      twist periodicity
+ *   observables, Fock expansion,       literal operators in the full Fock space applied to the
+ *   position operator                  embedded sector state; the charge-pump notebook's
+ *                                      stored charges, gaps and Chern number
+ *   phase refresh                      equal to a catalog rebuilt at the new phases
+ *   sampled rows, basis checker        equal to the projection table on small sectors; planted
+ *                                      defects in a basis are each reported
  *
  * Complex products are evaluated as the reference writes them, left to right;
  * build with -ffp-contract=off so no FMA contraction changes the rounding.
