@@ -116,7 +116,10 @@ int sed_catalog_destroy(sed_catalog* cat);
 
 /* build_symmetry_sector_basis(catalog, irrep; atol) (:544-556) with the irrep's
  * characters chi (n_group ComplexF64, OneDim_Irrep.values :501-504).  The catalog
- * must outlive the sector. */
+ * must outlive the sector.  Sectors hold at most 2^31 - 1 rows; one whose row index does
+ * not fit beside ceil(log2 n_group) + 1 group and sign bits in 4 bytes (more than 32 sites
+ * only) keeps an 8-byte entry per rank, and is refused with SED_EUNSUPPORTED when the
+ * catalog's site phases are not of gauge form. */
 int sed_sector_build(sed_catalog* cat, const double* chi, double atol, sed_sector** out, int64_t* dim);
 /* representative_mask_list and stabilizer_order_list of the sector basis (:522-528). */
 int sed_sector_basis(const sed_sector* sec, uint64_t* reps, int64_t* stab_order);
