@@ -14,6 +14,7 @@ behaviour (exceptions on failure):
     ed_scan!                       :1297-1356   (ed_scan; flux mode also as ed_scan_flux,
                                                  batched through the server)
     ed_scan_at_irrep_matrix!       :1061-1090   (ed_scan_at_irrep_matrix)
+    resume_ed_scan!                :1266-1277   (resume_ed_scan; checkpoints in checkpoint.py)
 
 Every call runs on the GPU through the HIP library; there is no CPU path here.
 """
@@ -40,7 +41,7 @@ __all__ = [
     "SectorServer", "PREC_FP64", "PREC_FP32", "PREC_FP32_REFINE", "sector_density",
     "density_correlation_matrix", "off_diagonal_long_range_order", "build_ed_Hamiltonian_symmetry_block",
     "expand_sector_state_to_fock_amplitudes", "fock_amplitudes", "position_operator_matrix",
-    "update_orbit_stabilizer_phases", "ed_scan", "ed_scan_at_irrep_matrix",
+    "update_orbit_stabilizer_phases", "ed_scan", "ed_scan_at_irrep_matrix", "resume_ed_scan",
 ]
 
 _default_ctx = None
