@@ -233,6 +233,65 @@ __global__ void __launch_bounds__(kRedBlock) k_finalize(int* step, const double2
     pro_advance(pro, j, a, b, beta);
 }
 
+// The whole step after the matvec in one workgroup, for sectors of at most kTailMaxRows
+// rows (their passes read tens of kilobytes, so one kernel with workgroup barriers beats
+// the eight launches of the multi-workgroup sequence: XXZ N=16 and 18 solves 17 % and 9 %
+// faster; from about 9,000 rows one CU's bandwidth makes it slower): the first Gram-Schmidt pass
+// over v_lo..v_j on w in place, pro_decide, the second pass over the whole basis when it
+// asks for it, alpha_j, beta_j and the breakdown flag, the step counter, and
+// v_{j+1} = w' / beta_j with its xhat.  The same arithmetic as k_dots .. k_scale, with the
+// reductions in one workgroup's fixed order.
+constexpr int kTailThreads = 1024;
+constexpr int64_t kTailMaxRows = 1 << 12;
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads) k_step_tail(T* __restrict__ V, int64_t n, int* step, T* __restrict__ w,
+                                                            double* __restrict__ alpha, double* __restrict__ beta,
+                                                            int* __restrict__ flag, ProState* __restrict__ pro,
+                                                            const uint8_t* __restrict__ stab, T* __restrict__ xhat) {
+    constexpr int NT = kTailThreads;
+    __shared__ double2 sh_h[kMaxBasis + 2];
+    __shared__ double2 sh_red[kBlockDots][NT / kWave];
+    __shared__ double sh_redd[NT / kWave];
+    __shared__ double sh_nrm;
+    __shared__ int sh_j, sh_lo, sh_need;
+    if (threadIdx.x == 0) {
+        sh_j = *step;
+        sh_lo = pro->lo;
+    }
+    __syncthreads();
+    const int j = sh_j, lo = sh_lo, nv = j + 1;
+    block_dots<T, NT>(V + (int64_t)lo * n, n, nv - lo, w, sh_h + lo, &sh_nrm, sh_red);
+    const double nw = sh_nrm;
+    double a = sh_h[j].x;
+    double nw2 = block_update<T, NT>(V + (int64_t)lo * n, n, nv - lo, sh_h + lo, w, sh_redd);
+    if (threadIdx.x == 0) sh_need = pro_decide(pro, j, nw2, nw, a, alpha, beta) ? 1 : 0;
+    __syncthreads();
+    if (sh_need) {
+        block_dots<T, NT>(V, n, nv, w, sh_h, &sh_nrm, sh_red);
+        a += sh_h[j].x;
+        nw2 = block_update<T, NT>(V, n, nv, sh_h, w, sh_redd);
+    }
+    const double b = sqrt(nw2);
+    const bool brk = !(b > 1e-12 * sqrt(nw)) || !(b > 0.0);
+    const double sc = brk ? 0.0 : 1.0 / b;
+    if (threadIdx.x == 0) {
+        alpha[j] = a;
+        beta[j] = b;
+        if (brk && *flag == 0) *flag = j + 1;
+        *step = j + 1;
+        pro_advance(pro, j, a, b, beta);
+    }
+    T* vn = V + (int64_t)(j + 1) * n;
+    for (int64_t r = threadIdx.x; r < n; r += NT) {
+        const double2 d = to_d2(w[r]);
+        const T o = from_d2<T>(make_double2(d.x * sc, d.y * sc));
+        vn[r] = o;
+        const double2 od = to_d2(o);
+        const double s = sqrt((double)stab[r]);
+        xhat[r] = from_d2<T>(make_double2(od.x * s, od.y * s));
+    }
+}
+
 // v_j *= scale and xhat = sqrt(stab) v_j for j = *step (the vector the next step applies H to).
 template <typename T>
 __global__ void __launch_bounds__(256) k_scale(T* V, int64_t n, const int* __restrict__ step,
@@ -492,7 +551,8 @@ static bool graphs_enabled() {
 // SED_SOLVE_EVENTS=1: every Lanczos loop launches its steps eagerly with HIP events between
 // the parts of a step and prints, at its end, the GPU time of each part summed over the
 // steps (H|psi>, first Gram-Schmidt pass with its check, gated second pass, finalisation and
-// scaling, gaps between consecutive steps), and the wall time of the thick restarts and of
+// scaling, gaps between consecutive steps; sectors of at most kTailMaxRows rows count their
+// one-workgroup step tail as the first pass), and the wall time of the thick restarts and of
 // the host checks, to stderr.  Read once per loop.
 static bool solve_events_enabled() {
     const char* e = std::getenv("SED_SOLVE_EVENTS");
@@ -713,6 +773,17 @@ class ThickLanczos {
         if (ev) SED_HIP(hipEventRecord(ev[0], st_));
         SED_TRY(launch_matvec_xhat(sec_, xh_, w_, VecTraits<T>::fp64, st_));
         if (ev) SED_HIP(hipEventRecord(ev[1], st_));
+        if (n_ <= kTailMaxRows) {
+            hipLaunchKernelGGL(k_step_tail<T>, dim3(1), dim3(kTailThreads), 0, st_, V_, (int64_t)n_, step_, w_, alpha_,
+                               beta_, flag_, pro_, sec_->d_stab.p, xh_);
+            if (ev) {
+                SED_HIP(hipEventRecord(ev[2], st_));
+                SED_HIP(hipEventRecord(ev[3], st_));
+                SED_HIP(hipEventRecord(ev[4], st_));
+            }
+            SED_HIP(hipGetLastError());
+            return SED_OK;
+        }
         // first classical Gram-Schmidt pass over v_lo..v_j (lo decided on the device) into
         // v_{j+1}; a second pass over the whole basis, in place, only when the device
         // decides so (partial re-orthogonalisation or cancellation)

@@ -39,7 +39,6 @@ constexpr int64_t kPackMaxDim = 1 << 15;   // one CU per sector: larger sectors 
 // The packed kernel always uses partial re-orthogonalisation: requests for more eigenpairs
 // than it covers go to the lanes.
 constexpr int kPackMaxNev = kProMaxNev;
-constexpr int kPackDots = 4;              // basis vectors per sweep of the dot products
 constexpr int kPackBatch = 4;             // hops whose table reads (then gathers) issue together
                                           // (divides the term padding; flux scan, FP64 batched:
                                           // 1287-1305 solves/s, 1225 with the unpadded loop)
@@ -83,101 +82,12 @@ struct PackMove {
     int jm, kk;
 };
 
-template <int NT>
-__device__ __forceinline__ double block_sum(double v, double* red) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
-    __syncthreads();
-    double s = 0.0;
-    for (int k = 0; k < NT / kWave; ++k) s += red[k];
-    __syncthreads();
-    return s;
-}
-
-// h[i] = V_i^dag w for i < nv and *nrm = |w|^2, kPackDots vectors per sweep over the
-// rows (results in LDS for every thread).
-template <typename T, int NT>
-__device__ __forceinline__ void pack_dots(const T* __restrict__ V, int64_t n, int nv, const T* __restrict__ w, double2* h,
-                          double* nrm, double2 (*red)[NT / kWave]) {
-    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    for (int g0 = 0; g0 <= nv; g0 += kPackDots) {
-        double2 acc[kPackDots];
-#pragma unroll
-        for (int q = 0; q < kPackDots; ++q) acc[q] = make_double2(0.0, 0.0);
-        for (int64_t r = threadIdx.x; r < n; r += NT) {
-            const double2 wr = to_d2(w[r]);
-            double2 v[kPackDots];
-#pragma unroll
-            for (int q = 0; q < kPackDots; ++q)
-                v[q] = g0 + q < nv ? to_d2(V[(int64_t)(g0 + q) * n + r]) : make_double2(0.0, 0.0);
-#pragma unroll
-            for (int q = 0; q < kPackDots; ++q) {
-                if (g0 + q < nv) {
-                    acc[q].x += v[q].x * wr.x + v[q].y * wr.y;
-                    acc[q].y += v[q].x * wr.y - v[q].y * wr.x;
-                } else if (g0 + q == nv) {
-                    acc[q].x += wr.x * wr.x + wr.y * wr.y;
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kPackDots; ++q) {
-            const double2 sm = wave_sum_d2(acc[q]);
-            if (lane == 0) red[q][wv] = sm;
-        }
-        __syncthreads();
-        if (threadIdx.x < kPackDots) {
-            const int i = g0 + threadIdx.x;
-            double2 sm = make_double2(0.0, 0.0);
-            for (int k = 0; k < NT / kWave; ++k) {
-                sm.x += red[threadIdx.x][k].x;
-                sm.y += red[threadIdx.x][k].y;
-            }
-            if (i < nv) h[i] = sm;
-            else if (i == nv) *nrm = sm.x;
-        }
-        __syncthreads();
-    }
-}
-
-// w <- w - sum_{i<nv} V_i h_i (four basis vectors per group of loads); returns |w|^2.
-template <typename T, int NT>
-__device__ __forceinline__ double pack_update(const T* __restrict__ V, int64_t n, int nv, const double2* h, T* w, double* red) {
-    double nrm = 0.0;
-    for (int64_t r = threadIdx.x; r < n; r += NT) {
-        double2 a = to_d2(w[r]);
-        int i = 0;
-        for (; i + 4 <= nv; i += 4) {
-            double2 v[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) v[q] = to_d2(V[(int64_t)(i + q) * n + r]);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const double2 c = h[i + q];
-                a.x -= v[q].x * c.x - v[q].y * c.y;
-                a.y -= v[q].x * c.y + v[q].y * c.x;
-            }
-        }
-        for (; i < nv; ++i) {
-            const double2 v = to_d2(V[(int64_t)i * n + r]);
-            const double2 c = h[i];
-            a.x -= v.x * c.x - v.y * c.y;
-            a.y -= v.x * c.y + v.y * c.x;
-        }
-        const T o = from_d2<T>(a);
-        w[r] = o;
-        const double2 od = to_d2(o);
-        nrm += od.x * od.x + od.y * od.y;
-    }
-    return block_sum<NT>(nrm, red);
-}
-
 template <typename M, typename R, bool FERMI>
 __global__ void __launch_bounds__(kPackBlock) k_pack_lanczos(const PackItem* __restrict__ items) {
     using T = typename C2<R>::T;
     constexpr int NT = kPackBlock;
     __shared__ double2 sh_h[kMaxBasis + 2];
-    __shared__ double2 sh_red[kPackDots][NT / kWave];
+    __shared__ double2 sh_red[kBlockDots][NT / kWave];
     __shared__ double sh_redd[NT / kWave];
     __shared__ double sh_nrm;
     __shared__ int sh_lo, sh_need;
@@ -254,16 +164,16 @@ __global__ void __launch_bounds__(kPackBlock) k_pack_lanczos(const PackItem* __r
         // first pass over v_lo..v_j, then the whole basis when pro_decide asks for it
         const int lo = sh_lo;
         const int nv = j + 1;
-        pack_dots<T, NT>(V + (int64_t)lo * n, n, nv - lo, w, sh_h + lo, &sh_nrm, sh_red);
+        block_dots<T, NT>(V + (int64_t)lo * n, n, nv - lo, w, sh_h + lo, &sh_nrm, sh_red);
         const double nw = sh_nrm;
         double a = sh_h[j].x;
-        double nw2 = pack_update<T, NT>(V + (int64_t)lo * n, n, nv - lo, sh_h + lo, w, sh_redd);
+        double nw2 = block_update<T, NT>(V + (int64_t)lo * n, n, nv - lo, sh_h + lo, w, sh_redd);
         if (threadIdx.x == 0) sh_need = pro_decide(pro, j, nw2, nw, a, alpha, beta) ? 1 : 0;
         __syncthreads();
         if (sh_need) {
-            pack_dots<T, NT>(V, n, nv, w, sh_h, &sh_nrm, sh_red);
+            block_dots<T, NT>(V, n, nv, w, sh_h, &sh_nrm, sh_red);
             a += sh_h[j].x;
-            nw2 = pack_update<T, NT>(V, n, nv, sh_h, w, sh_redd);
+            nw2 = block_update<T, NT>(V, n, nv, sh_h, w, sh_redd);
         }
         const double b = sqrt(nw2);
         const bool brk = !(b > 1e-12 * sqrt(nw)) || !(b > 0.0);

@@ -213,4 +213,101 @@ __device__ __forceinline__ double2 gauss_entry(uint64_t seed, int64_t r) {
     return make_double2(rad * cos(th), rad * sin(th));
 }
 
+// ---------------------------------------------------------------------------
+// Gram-Schmidt passes inside one workgroup (the packed engine, and the fused step tail of
+// sectors small enough for one workgroup)
+// ---------------------------------------------------------------------------
+constexpr int kBlockDots = 4;   // basis vectors per sweep of the dot products
+
+// Sum of v over a workgroup of NT threads, returned to every thread (fixed order).
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < NT / kWave; ++k) s += red[k];
+    __syncthreads();
+    return s;
+}
+
+// One workgroup of NT threads: h[i] = V_i^dag w for i < nv and *nrm = |w|^2, kBlockDots
+// vectors per sweep over the rows (results in LDS for every thread).
+template <typename T, int NT>
+__device__ __forceinline__ void block_dots(const T* __restrict__ V, int64_t n, int nv, const T* __restrict__ w, double2* h,
+                          double* nrm, double2 (*red)[NT / kWave]) {
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    for (int g0 = 0; g0 <= nv; g0 += kBlockDots) {
+        double2 acc[kBlockDots];
+#pragma unroll
+        for (int q = 0; q < kBlockDots; ++q) acc[q] = make_double2(0.0, 0.0);
+        for (int64_t r = threadIdx.x; r < n; r += NT) {
+            const double2 wr = to_d2(w[r]);
+            double2 v[kBlockDots];
+#pragma unroll
+            for (int q = 0; q < kBlockDots; ++q)
+                v[q] = g0 + q < nv ? to_d2(V[(int64_t)(g0 + q) * n + r]) : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < kBlockDots; ++q) {
+                if (g0 + q < nv) {
+                    acc[q].x += v[q].x * wr.x + v[q].y * wr.y;
+                    acc[q].y += v[q].x * wr.y - v[q].y * wr.x;
+                } else if (g0 + q == nv) {
+                    acc[q].x += wr.x * wr.x + wr.y * wr.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kBlockDots; ++q) {
+            const double2 sm = wave_sum_d2(acc[q]);
+            if (lane == 0) red[q][wv] = sm;
+        }
+        __syncthreads();
+        if (threadIdx.x < kBlockDots) {
+            const int i = g0 + threadIdx.x;
+            double2 sm = make_double2(0.0, 0.0);
+            for (int k = 0; k < NT / kWave; ++k) {
+                sm.x += red[threadIdx.x][k].x;
+                sm.y += red[threadIdx.x][k].y;
+            }
+            if (i < nv) h[i] = sm;
+            else if (i == nv) *nrm = sm.x;
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup: w <- w - sum_{i<nv} V_i h_i (four basis vectors per group of loads);
+// returns |w|^2 to every thread.
+template <typename T, int NT>
+__device__ __forceinline__ double block_update(const T* __restrict__ V, int64_t n, int nv, const double2* h, T* w, double* red) {
+    double nrm = 0.0;
+    for (int64_t r = threadIdx.x; r < n; r += NT) {
+        double2 a = to_d2(w[r]);
+        int i = 0;
+        for (; i + 4 <= nv; i += 4) {
+            double2 v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = to_d2(V[(int64_t)(i + q) * n + r]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double2 c = h[i + q];
+                a.x -= v[q].x * c.x - v[q].y * c.y;
+                a.y -= v[q].x * c.y + v[q].y * c.x;
+            }
+        }
+        for (; i < nv; ++i) {
+            const double2 v = to_d2(V[(int64_t)i * n + r]);
+            const double2 c = h[i];
+            a.x -= v.x * c.x - v.y * c.y;
+            a.y -= v.x * c.y + v.y * c.x;
+        }
+        const T o = from_d2<T>(a);
+        w[r] = o;
+        const double2 od = to_d2(o);
+        nrm += od.x * od.x + od.y * od.y;
+    }
+    return block_sum<NT>(nrm, red);
+}
+
 }  // namespace sed
