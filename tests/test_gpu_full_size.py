@@ -194,6 +194,34 @@ def test_full_size_eigenpairs_satisfy_the_oracle_rows(ed, oracle_mod, full, name
         assert np.abs(v - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("name", ["xxz34", "hubbard4x4", "checker4x6"])
+def test_full_size_observables_reproduce_the_energy(ed, full, name):
+    """Observables of the ground state at full size, including the 8-byte-table sector of
+    XXZ N=34: the one-body matrix and <n_i n_j> reproduce the eigenvalue through the model's
+    own terms, E = sum_k t_k <a^dag_to a_from> + sum_d V_d <n_i n_j> (1e-9 relative); the
+    one-body matrix is Hermitian with the site densities on its diagonal and trace N; the
+    densities are translation invariant per sublattice orbit and sum to N; and
+    sum_j <n_i n_j> = N <n_i>."""
+    fs = full(name, (0.0, 0.0))
+    m = fs.m
+    vals, _ = ed.diagonalize_block_matrixfree(fs.H, fs.b.dim, nev=1)
+    e0 = float(vals[0])
+    d = ed.sector_density(fs.H, 0)
+    C, d2 = ed.density_correlation_matrix(fs.H, 0, subtract_disconnected=False)
+    rho = ed.off_diagonal_long_range_order(fs.H, 0)
+    k = m.n_filled
+    assert np.abs(d - d2).max() < 1e-12 and abs(d.sum() - k) < 1e-9
+    assert np.abs(np.diag(rho).real - d).max() < 1e-10 and abs(np.trace(rho) - k) < 1e-9
+    assert np.abs(rho - rho.conj().T).max() < 1e-12
+    assert np.abs(C.sum(axis=1) - k * d).max() < 1e-9
+    for g in range(fs.G.order):   # a translation-invariant state: <n_g(i)> = <n_i>
+        assert np.abs(d[fs.G.perm[g] - 1] - d).max() < 1e-10
+    e = sum(complex(t) * rho[b - 1, a - 1] for a, b, t in m.bilinear_terms)
+    e += sum(complex(v) * C[i - 1, j - 1] for i, j, v in m.density_terms)
+    print(f"[full-obs] {name} E0={e0:.12f} from observables {e.real:.12f} (imag {e.imag:.1e})", flush=True)
+    assert abs(e.imag) < 1e-9 * abs(e0) and abs(e.real - e0) < 1e-9 * abs(e0)
+
+
 def test_xxz34_every_launch_shape_on_the_8_byte_table(ed, oracle_mod, full):
     """The k = 0 sector of XXZ N=34 (68,635,478 rows at |G| = 34) does not fit the 4-byte
     table entry (row < 2^25 beside 7 group and parity bits) and runs on the 8-byte table:
