@@ -120,9 +120,9 @@ def cpu_solves(threads):
     sector basis, serial projection-table build (:680-728), Lanczos eigensolve (ARPACK
     through scipy over _apply_table!, :731-765, with `threads` per-thread buffers; the
     reference uses KrylovKit) at tol 1e-10; the catalog is built before the timed region,
-    as the reference's benchmark does (benchmark/benchmark.jl:226-241).  Config[0]'s sector
-    (XXZ N=16, k=0, dim 810, nev 4) and config[3]'s (Haldane 3x4 bosons at zero twist,
-    k=(0,0), dim 11,240), the latter at nev 1 beside the reference's published 0.872 s
+    as the reference's benchmark does (benchmark/benchmark.jl:226-241).  The XXZ N=16 sector
+    (k=0, dim 810, nev 4) and the flux scan's Haldane 3x4 boson sector at zero twist
+    (k=(0,0), dim 11,240), the latter at nev 1 beside the reference's published 0.872 s
     (benchmark/benchmark_data/benchmark_raw.csv:20-21, hardware unrecorded)."""
     import numpy as np
     from scipy.sparse.linalg import LinearOperator, eigsh
@@ -197,7 +197,7 @@ def cpu_baseline(model, G, ir, cat, basis, seconds, threads, bytes_c64, gated, i
 
 
 def flux_scan(E, L, ctx, n_twists=16, nev=3, lanes=8):
-    """Config[3]-style flux scan: Haldane-boson 3x4 (24 sites, 6 bosons), all 12 momentum
+    """Flux scan: Haldane-boson 3x4 (24 sites, 6 bosons), all 12 momentum
     sectors x n_twists twists theta_x in [0, 1), lowest nev eigenvalues each.
 
     Three ways: the plain FP64 path one solve after another (sed_solve), and the
@@ -267,7 +267,7 @@ def poisson_arrivals(n, rate, seed):
 
 
 def serving_stream(E, L, ctx, n_req=32, seed=2024, nev=3, lanes=8, load=1.2):
-    """Config[4]-style serving stream.  The sectors of all requests are built first (tables
+    """Serving stream of mixed sector solves.  The sectors of all requests are built first (tables
     resident in HBM) and every model's matvec launch shape is tuned in both precisions (a
     one-time cost per model, reported apart, so that no timed path below pays it); then
     (1) the plain FP64 path solves them one after another, (2) the continuous-batching

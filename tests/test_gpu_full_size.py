@@ -12,7 +12,7 @@ rests on the table-free oracle and on size-independent properties:
     residuals, orthonormality, other seeds and FP32+refine against FP64;
   * sector dimensions against the character formula (closed form C1) and their sum
     against C(N, k); Hermiticity and linearity of the matvec; FP32 against FP64.
-Config[0] (XXZ N=16, dim 810) is small enough for the oracle's dense spectrum.
+The XXZ N=16 sector (dim 810) is small enough for the oracle's dense spectrum.
 """
 import math
 
