@@ -27,6 +27,9 @@ give workload_bytes_per_launch, reported beside it; they count bytes this layout
 move, so their rate is not an HBM fraction.  roofline.traffic is the HBM traffic per
 launch measured by rocprofv3 counters (FETCH_SIZE and WRITE_SIZE, copy-calibrated) in
 the committed profile named in roofline.traffic_source (null for other workloads).
+hbm_streaming gives the rates of two plain streaming kernels over 2 GiB measured in the
+same run (torch's device copy, read + write, and sum, read only): what library streaming
+kernels reach on this GPU, beside the published peak.  matvecs_per_s = 1000 / ms_per_step.
 """
 import argparse
 import json
@@ -90,6 +93,49 @@ def measured_traffic(precision, workload):
     if val is None:
         return None, None
     return val, f"{COUNTER_PROFILE} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated; {src or 'build unstated'})"
+
+
+_COPY_PROBE = r"""
+import json, sys, torch
+gib, reps = int(sys.argv[1]), int(sys.argv[2])
+n = (gib << 30) // 4
+a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+b = torch.empty_like(a)
+for _ in range(3):
+    b.copy_(a)
+e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+e0.record()
+for _ in range(reps):
+    b.copy_(a)
+e1.record()
+e1.synchronize()
+ms = e0.elapsed_time(e1) / reps
+for _ in range(3):
+    s = a.sum()
+e0.record()
+for _ in range(reps):
+    s = a.sum()
+e1.record()
+e1.synchronize()
+ms_sum = e0.elapsed_time(e1) / reps
+print(json.dumps({"copy_GB/s": round(2 * 4 * n / (ms * 1e-3) / 1e9, 1), "copy_bytes": 2 * 4 * n,
+                  "copy_ms": round(ms, 4), "sum_GB/s": round(4 * n / (ms_sum * 1e-3) / 1e9, 1),
+                  "sum_bytes": 4 * n, "sum_ms": round(ms_sum, 4)}))
+"""
+
+
+def copy_bandwidth(gib=2, reps=10):
+    """Rates (GB/s) of two plain streaming kernels over `gib` GiB of float32, timed with
+    HIP events on torch's stream: a device-to-device copy (`copy_`, read + write) and a
+    sum (read only).  Reported beside the published 8 TB/s peak as what library streaming
+    kernels reach on this GPU.  Measured in a child process (torch brings its own HIP
+    runtime, which cannot share this process with the library's)."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-c", _COPY_PROBE, str(gib), str(reps)], capture_output=True, text=True,
+                       timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(f"copy probe failed:\n{r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def host_info():
@@ -513,6 +559,11 @@ def main():
                           "dispatches": 3}))
         return
 
+    # before this process touches the GPU: the probe is a child process
+    copy = copy_bandwidth()
+    log(f"[bench] streaming kernels: copy {copy['copy_GB/s']:.0f} GB/s (read + write), "
+        f"sum {copy['sum_GB/s']:.0f} GB/s (read)", flush=True)
+
     from realspace_exactdiagonalization_amd import ed as E
     from realspace_exactdiagonalization_amd import lattice as L
 
@@ -589,6 +640,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(main_r["ms"], 4),
+        "matvecs_per_s": round(1e3 / main_r["ms"], 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -603,7 +655,9 @@ def main():
                                       "c32_GB/s": round(res["c32"]["workload_gbs"], 2),
                                       "c64_GB/s": round(res["c64"]["workload_gbs"], 2)},
         "cpu_baseline": cpu,
-        "fp32": {"ms_per_matvec": round(res["c32"]["ms"], 4), "algorithmic_bytes_per_launch": res["c32"]["bytes"],
+        "hbm_streaming": copy,
+        "fp32": {"ms_per_matvec": round(res["c32"]["ms"], 4), "matvecs_per_s": round(1e3 / res["c32"]["ms"], 1),
+                 "algorithmic_bytes_per_launch": res["c32"]["bytes"],
                  "roofline": roof(res["c32"]), "launch_shape": res["c32"]["shape"]},
         "launch_shape": res["c64"]["shape"],
         "flux_scan_haldane3x4": flux,
