@@ -29,7 +29,8 @@ launch measured by rocprofv3 counters (FETCH_SIZE and WRITE_SIZE, copy-calibrate
 the committed profile named in roofline.traffic_source (null for other workloads).
 hbm_streaming gives the rates of two plain streaming kernels over 2 GiB measured in the
 same run (torch's device copy, read + write, and sum, read only): what library streaming
-kernels reach on this GPU, beside the published peak.  matvecs_per_s = 1000 / ms_per_step.
+kernels reach on this GPU, beside the published peak (null under rocprofv3, where this
+process starts no child process).  matvecs_per_s = 1000 / ms_per_step.
 """
 import argparse
 import json
@@ -131,6 +132,10 @@ def copy_bandwidth(gib=2, reps=10):
     kernels reach on this GPU.  Measured in a child process (torch brings its own HIP
     runtime, which cannot share this process with the library's)."""
     import subprocess
+    if any(k.startswith("ROCPROF") for k in os.environ):
+        # under rocprofv3 this process is set up for the GPU before it starts; it then
+        # starts no other program
+        return None
     r = subprocess.run([sys.executable, "-c", _COPY_PROBE, str(gib), str(reps)], capture_output=True, text=True,
                        timeout=300)
     if r.returncode != 0:
@@ -561,8 +566,9 @@ def main():
 
     # before this process touches the GPU: the probe is a child process
     copy = copy_bandwidth()
-    log(f"[bench] streaming kernels: copy {copy['copy_GB/s']:.0f} GB/s (read + write), "
-        f"sum {copy['sum_GB/s']:.0f} GB/s (read)", flush=True)
+    if copy:
+        log(f"[bench] streaming kernels: copy {copy['copy_GB/s']:.0f} GB/s (read + write), "
+            f"sum {copy['sum_GB/s']:.0f} GB/s (read)", flush=True)
 
     from realspace_exactdiagonalization_amd import ed as E
     from realspace_exactdiagonalization_amd import lattice as L
