@@ -83,8 +83,15 @@ def lib():
         L.oc_project_to_unnormalized_sector.argtypes = [C.c_void_p, C.c_uint64, _f64p]
         L.oc_table_build.restype = C.c_int64
         L.oc_table_build.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _i32p, _f64p]
+        L.oc_table_build_cols.restype = C.c_int64
+        L.oc_table_build_cols.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _i32p, _f64p,
+                                          C.c_int64, C.c_int64]
+        L.oc_table_build_strided.restype = C.c_int64
+        L.oc_table_build_strided.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _i32p,
+                                             _f64p, C.c_int64, C.c_int64, C.c_int64, C.c_int64]
         L.oc_table_copy.argtypes = [C.c_void_p, _i64p, _i64p, _f64p, _f64p]
         L.oc_apply_table.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int]
+        L.oc_apply_table_timed.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int, _f64p]
         L.oc_apply_hamiltonian.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _i32p,
                                            _f64p, _f64p, _f64p, C.c_int, C.c_int64, C.c_int64]
         L.oc_density_correlation.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
@@ -269,6 +276,14 @@ class Sector:
         self.nnz = int(lib().oc_table_build(self._h, len(a), a, b, tt, len(c), c, d, vv))
         return self.nnz
 
+    def build_table_cols(self, c0: int, c1: int, period: int = 1, block: int = 1) -> int:
+        """The projection table of the columns c of [c0, c1) with (c - c0) mod period <
+        block only (replacing any table built before); returns its number of entries."""
+        a, b, tt, c, d, vv = self._terms
+        self.nnz = int(lib().oc_table_build_strided(self._h, len(a), a, b, tt, len(c), c, d, vv, int(c0), int(c1),
+                                                    int(period), int(block)))
+        return self.nnz
+
     def table(self):
         """``build_matrixfree_projection_table`` output: (row, col) 1-based, vals, h_diag."""
         row = np.zeros(self.nnz, dtype=np.int64)
@@ -284,6 +299,15 @@ class Sector:
         y = np.zeros(self.dim, dtype=np.complex128)
         lib().oc_apply_table(self._h, x.view(np.float64), y.view(np.float64), nthreads)
         return y
+
+    def apply_table_timed(self, x, nthreads: int = 1):
+        """``_apply_table!`` and the wall seconds of its parts: (y, {"zero", "diagonal",
+        "entries", "reduction"})."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros(self.dim, dtype=np.complex128)
+        ph = np.zeros(4)
+        lib().oc_apply_table_timed(self._h, x.view(np.float64), y.view(np.float64), nthreads, ph)
+        return y, dict(zip(("zero", "diagonal", "entries", "reduction"), ph.tolist()))
 
     def apply_hamiltonian(self, x, nthreads: int = 1, c0: int = 0, c1: Optional[int] = None) -> np.ndarray:
         """On-the-fly ``apply_hamiltonian!`` over columns [c0, c1)."""

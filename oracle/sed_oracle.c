@@ -58,6 +58,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -384,6 +385,10 @@ typedef struct {
     int64_t* col;         /* 1-based */
     cplx* vals;
     cplx* hdiag;
+    /* _apply_table!'s per-thread buffers (MatrixFreeHamiltonian's y_threads), kept
+     * between calls as the reference keeps them */
+    cplx* ybuf;
+    int ybuf_threads;
 } oc_sector;
 
 static int is_orbit_compatible(const oc_catalog* c, int64_t o, const cplx* chi, double atol) {
@@ -438,6 +443,7 @@ void oc_sector_free(oc_sector* s) {
     if (!s) return;
     free(s->chi); free(s->reps); free(s->stab);
     free(s->row); free(s->col); free(s->vals); free(s->hdiag);
+    free(s->ybuf);
     free(s);
 }
 
@@ -508,9 +514,14 @@ static inline double hop_phase(int fermionic, mask_t m, int i_from, int i_to) {
     return (__builtin_popcountll(m & between) & 1) ? -1.0 : 1.0;
 }
 
-/* build_matrixfree_projection_table (:680-728); returns nnz */
-int64_t oc_table_build(oc_sector* s, int n_bil, const int32_t* ifrom, const int32_t* ito, const double* t_,
-                       int n_den, const int32_t* di, const int32_t* dj, const double* V_) {
+/* build_matrixfree_projection_table (:680-728) over the columns c of [c0, c1) with
+ * (c - c0) mod period < block, in ascending order (the reference builds all of them:
+ * c0 = 0, c1 = dim, period = block = 1; runs of columns spread over the sector let a host
+ * baseline time the serial build and _apply_table! on a bounded sample of a sector whose
+ * whole table would take minutes); h_diag is filled for those columns.  Returns nnz. */
+int64_t oc_table_build_strided(oc_sector* s, int n_bil, const int32_t* ifrom, const int32_t* ito,
+                               const double* t_, int n_den, const int32_t* di, const int32_t* dj,
+                               const double* V_, int64_t c0, int64_t c1, int64_t period, int64_t block) {
     const cplx* t = (const cplx*)t_;
     const cplx* V = (const cplx*)V_;
     const oc_catalog* c = s->cat;
@@ -526,7 +537,15 @@ int64_t oc_table_build(oc_sector* s, int n_bil, const int32_t* ifrom, const int3
         hop[k] = ((mask_t)1 << (ifrom[k] - 1)) | ((mask_t)1 << (ito[k] - 1));
         from[k] = (mask_t)1 << (ifrom[k] - 1);
     }
-    for (int64_t col = 0; col < s->dim; ++col) {
+    if (c0 < 0) c0 = 0;
+    if (c1 > s->dim) c1 = s->dim;
+    if (period < 1) period = 1;
+    if (block < 1) block = 1;
+    for (int64_t col = c0; col < c1; ++col) {
+        if ((col - c0) % period >= block) {
+            col += period - (col - c0) % period - 1;   /* to the next run */
+            continue;
+        }
         mask_t m = s->reps[col];
         int64_t stab_col = s->stab[col];
         cplx hd = 0.0;
@@ -558,6 +577,18 @@ int64_t oc_table_build(oc_sector* s, int n_bil, const int32_t* ifrom, const int3
     return nnz;
 }
 
+int64_t oc_table_build_cols(oc_sector* s, int n_bil, const int32_t* ifrom, const int32_t* ito, const double* t_,
+                            int n_den, const int32_t* di, const int32_t* dj, const double* V_, int64_t c0,
+                            int64_t c1) {
+    return oc_table_build_strided(s, n_bil, ifrom, ito, t_, n_den, di, dj, V_, c0, c1, 1, 1);
+}
+
+/* build_matrixfree_projection_table (:680-728); returns nnz */
+int64_t oc_table_build(oc_sector* s, int n_bil, const int32_t* ifrom, const int32_t* ito, const double* t_,
+                       int n_den, const int32_t* di, const int32_t* dj, const double* V_) {
+    return oc_table_build_strided(s, n_bil, ifrom, ito, t_, n_den, di, dj, V_, 0, s->dim, 1, 1);
+}
+
 void oc_table_copy(const oc_sector* s, int64_t* row, int64_t* col, double* vals, double* hdiag) {
     if (row) memcpy(row, s->row, s->nnz * sizeof(int64_t));
     if (col) memcpy(col, s->col, s->nnz * sizeof(int64_t));
@@ -565,19 +596,37 @@ void oc_table_copy(const oc_sector* s, int64_t* row, int64_t* col, double* vals,
     if (hdiag) memcpy(hdiag, s->hdiag, s->dim * sizeof(cplx));
 }
 
+static double wall_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 /* _apply_table! (:731-765): per-thread dense buffers, serial diagonal, static edge
- * split over nthreads, serial reduction over the buffers. */
-void oc_apply_table(const oc_sector* s, const double* x_, double* y_, int nthreads) {
+ * split over nthreads, serial reduction over the buffers.  With phases != NULL the wall
+ * time of its four parts is stored there: buffer allocation and zeroing, diagonal,
+ * entries, reduction (so a host baseline can scale the entry part of a partial table). */
+void oc_apply_table_timed(oc_sector* s, const double* x_, double* y_, int nthreads, double* phases) {
     const cplx* x = (const cplx*)x_;
     cplx* y = (cplx*)y_;
     int64_t n = s->dim;
     if (nthreads < 1) nthreads = 1;
-    cplx* buf = calloc((size_t)nthreads * (n > 0 ? n : 1), sizeof(cplx));
+    if (!s->ybuf || s->ybuf_threads != nthreads) {
+        free(s->ybuf);
+        s->ybuf = calloc((size_t)nthreads * (n > 0 ? n : 1), sizeof(cplx));
+        s->ybuf_threads = nthreads;
+    }
+    cplx* buf = s->ybuf;
+    double t0 = wall_s();
+    /* fill!(y_threads[tid], 0) for every buffer, one after another (:739-741) */
+    memset(buf, 0, (size_t)nthreads * (n > 0 ? n : 1) * sizeof(cplx));
+    double t1 = wall_s();
     for (int64_t col = 0; col < n; ++col) {
         cplx xc = x[col];
         if (xc == 0) continue;
         buf[col] += s->hdiag[col] * xc;
     }
+    double t2 = wall_s();
     int64_t nnz = s->nnz;
 #pragma omp parallel num_threads(nthreads)
     {
@@ -594,12 +643,22 @@ void oc_apply_table(const oc_sector* s, const double* x_, double* y_, int nthrea
             yt[s->row[e] - 1] += s->vals[e] * xc;
         }
     }
+    double t3 = wall_s();
     for (int64_t i = 0; i < n; ++i) y[i] = 0;
     for (int t = 0; t < nthreads; ++t) {
         const cplx* yt = buf + (size_t)t * n;
         for (int64_t i = 0; i < n; ++i) y[i] += yt[i];
     }
-    free(buf);
+    if (phases) {
+        phases[0] = t1 - t0;
+        phases[1] = t2 - t1;
+        phases[2] = t3 - t2;
+        phases[3] = wall_s() - t3;
+    }
+}
+
+void oc_apply_table(oc_sector* s, const double* x_, double* y_, int nthreads) {
+    oc_apply_table_timed(s, x_, y_, nthreads, NULL);
 }
 
 /* apply_hamiltonian! (:813-878): on-the-fly, threaded over columns [c0, c1) */

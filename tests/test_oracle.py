@@ -326,6 +326,37 @@ def test_apply_table_matches_dense_and_on_the_fly(oracle_mod):
         assert np.abs(sec.apply_hamiltonian(x, nt) - y_dense).max() < 1e-10
 
 
+def test_table_of_a_column_range_is_that_slice_of_the_table(oracle_mod):
+    """The projection table of the runs of columns of [c0, c1) that the host baseline
+    samples (c - c0 mod period < block) is exactly those columns' entries of the whole
+    table, in the same order, and _apply_table! over it (timed or not) equals the
+    on-the-fly matvec restricted to those columns."""
+    m = L.twisted(L.checkerboard(3, 2, n_filled=4, V1=1.0, V2=0.4), (0.23, 0.17))
+    G = L.model_group(m, (0.23, 0.17))
+    ir = L.translation_irreps(G, m.lattice)[3]
+    _, sec = _sector(oracle_mod, m, G, ir.values)
+    n = sec.dim
+    row, col, vals, hd = sec.table()
+    x = np.arange(1, n + 1) + 1j
+    for c0, c1, per, blk in ((0, n, 1, 1), (0, n // 3, 1, 1), (n // 3, n, 1, 1), (n // 2, n // 2, 1, 1),
+                             (5, 9, 1, 1), (0, n, 7, 1), (3, n, 5, 1), (0, n, 10, 3), (2, n - 1, 9, 9)):
+        nnz = sec.build_table_cols(c0, c1, per, blk)
+        r2, k2, v2, hd2 = sec.table()
+        cs = np.array([c for c in range(c0, c1) if (c - c0) % per < blk], dtype=np.int64)
+        sel = np.isin(col - 1, cs)
+        assert nnz == int(sel.sum())
+        assert np.array_equal(r2, row[sel]) and np.array_equal(k2, col[sel]) and np.array_equal(v2, vals[sel])
+        assert np.array_equal(hd2[cs], hd[cs])
+        xs = np.zeros_like(x)
+        xs[cs] = x[cs]
+        for nt in (1, 4):
+            y_ref = sum((sec.apply_hamiltonian(x, nt, c, c + 1) for c in cs), np.zeros(n, dtype=complex))
+            assert np.abs(sec.apply_table(xs, nt) - y_ref).max() < 1e-10
+            y_t, ph = sec.apply_table_timed(xs, nt)
+            assert np.array_equal(y_t, sec.apply_table(xs, nt)) and set(ph) == {"zero", "diagonal", "entries",
+                                                                                 "reduction"}
+
+
 def test_project_to_sector_agrees_with_unnormalized(oracle_mod):
     """test/flux_regression.jl:36-44: table lookup == O(|G|) canonicalisation."""
     m = L.twisted(L.checkerboard(3, 2, n_filled=4), (0.23, 0.17))
