@@ -161,8 +161,9 @@ int sed_matvec_benchmark(sed_sector* sec, int precision, int n_calls, double* ms
  * order, 0 = one contiguous range per XCD).  shape = -1 re-enables the autotuner
  * (xcd_chunk ignored), which on the first matvec of a sector of dimension >= 2^17 times
  * every shape, then the run lengths for the fastest one, then the shapes again at the
- * fastest run length, and keeps the fastest pair (smaller sectors use shape 1, run
- * length 1). */
+ * fastest run length, and keeps the fastest pair; sectors of 512 <= dimension < 2^17 get
+ * the fastest shape at run length 1, each shape timed over back-to-back launches, and
+ * smaller ones shape 1 at run length 1. */
 int sed_sector_set_shape(sed_sector* sec, int precision, int shape, int xcd_chunk);
 /* Chosen shape (-1 if not chosen yet), its lanes/batch and XCD run length, the numbers
  * of shapes and of run lengths tried, the autotuner's times in ms (may be NULL;

@@ -102,6 +102,8 @@ static const MvShape kShapes[] = {{1, 1, 256, false}, {1, 4, 256, false}, {2, 4,
                                   {1, 4, 256, true},  {1, 8, 256, true},  {2, 4, 256, true},
                                   {2, 8, 256, true}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+// the shape of sectors too small to tune and of launches before any tuning
+constexpr int kDefaultShape = 1;
 // XCD run lengths the autotuner tries for the chosen shape (0 = one range per XCD)
 static const int kChunks[] = {1, 16, 64, 256, 1024, 0};
 constexpr int kNumChunks = sizeof(kChunks) / sizeof(kChunks[0]);
@@ -407,8 +409,8 @@ static int run_any_shape(sed_sector* sec, int shape, int chunk, const typename C
              : run_shape<uint64_t, R, false, uint32_t>(sec, shape, chunk, xh, y, s);
 }
 
-// Autotuner: on the first matvec of a sector large enough for the launch shape to
-// matter, time every shape (XCD run length 1) on the call's own buffers, then every XCD
+// Autotuner: on the first matvec of a sector of at least kTuneSmallDim rows, time every
+// shape (XCD run length 1) on the call's own buffers, then (from kTuneMinDim rows) every XCD
 // run length for the fastest shape, then the shapes that came within 20 % of the fastest
 // at the fastest run length (deterministic kernels, so the repeated launches leave y as
 // one launch would), and keep the fastest pair.  A candidate whose first timed launch is
@@ -419,23 +421,36 @@ static int run_any_shape(sed_sector* sec, int shape, int chunk, const typename C
 // skips re-tuning (all momentum sectors of a model share it).
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, int, int, int, int, int>, std::pair<int, int>> g_tune_cache;
+// Sectors from kTuneMinDim rows are tuned over shapes and XCD run lengths with single
+// launches; smaller ones (down to kTuneSmallDim rows) over the shapes only, at run length
+// 1 (their grids are a few workgroups per XCD), each candidate timed over back-to-back
+// launches (their kernels take microseconds, so one launch between two events would time
+// the launch rather than the kernel).  Below kTuneSmallDim rows the default shape runs.
 constexpr int64_t kTuneMinDim = 1 << 17;
+constexpr int64_t kTuneSmallDim = 512;
 
-// Time one (shape, run length): an optional warm-up launch, one timed launch, and two more
-// when the first is within 10 % of `best` (the best time so far); the minimum.
+// launches per timed sample: about kTuneMinDim rows' worth of work, at most 32
+static int tune_reps(int64_t dim) {
+    if (dim >= kTuneMinDim) return 1;
+    return (int)std::min<int64_t>(32, std::max<int64_t>(1, kTuneMinDim / std::max<int64_t>(dim, 1)));
+}
+
+// Time one (shape, run length): an optional warm-up launch, one timed sample, and two more
+// when the first is within 10 % of `best` (the best time so far); the minimum.  A sample
+// is `reps` back-to-back launches, reported per launch.
 template <typename R>
 static int time_shape(sed_sector* sec, int shape, int chunk, const typename C2<R>::T* xh, typename C2<R>::T* y,
-                      hipStream_t s, hipEvent_t e0, hipEvent_t e1, bool warm, float best, float* ms_out) {
+                      hipStream_t s, hipEvent_t e0, hipEvent_t e1, bool warm, float best, float* ms_out, int reps = 1) {
     if (warm) SED_TRY(run_any_shape<R>(sec, shape, chunk, xh, y, s));
     float mine = 1e30f;
     for (int k = 0; k < 3; ++k) {
         SED_HIP(hipEventRecord(e0, s));
-        SED_TRY(run_any_shape<R>(sec, shape, chunk, xh, y, s));
+        for (int r = 0; r < reps; ++r) SED_TRY(run_any_shape<R>(sec, shape, chunk, xh, y, s));
         SED_HIP(hipEventRecord(e1, s));
         SED_HIP(hipEventSynchronize(e1));
         float ms = 0;
         SED_HIP(hipEventElapsedTime(&ms, e0, e1));
-        mine = std::min(mine, ms);
+        mine = std::min(mine, ms / (float)reps);
         if (k == 0 && mine > 1.1f * best) break;
     }
     *ms_out = mine;
@@ -455,10 +470,12 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<
             return sec->mv_shape[pi] = v;
         }
     }
-    if (sec->dim < kTuneMinDim) {
+    if (sec->dim < kTuneSmallDim) {
         sec->mv_chunk[pi] = 1;
-        return sec->mv_shape[pi] = 1;
+        return sec->mv_shape[pi] = kDefaultShape;
     }
+    const bool small = sec->dim < kTuneMinDim;
+    const int reps = tune_reps(sec->dim);
     const auto key = std::make_tuple(sec->cat->n_site, sec->cat->n_filled, sec->cat->n_group, sec->cat->stats,
                                      sec->n_terms, (int)std::lround(4.0 * std::log2((double)sec->dim)), pi);
     {
@@ -473,21 +490,23 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<
     SED_HIP(hipEventCreate(&e0));
     SED_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    int best_i = 1, best_c = 1;
+    int best_i = kDefaultShape, best_c = 1;
     for (int i = 0; i < kNumShapes; ++i) {
         float ms = 0;
         if (!shape_fits<R>(sec, i)) {
             sec->tune_ms[pi][i] = -1.0f;
             continue;
         }
-        SED_TRY(time_shape<R>(sec, i, 1, xh, y, s, e0, e1, true, best, &ms));
+        SED_TRY(time_shape<R>(sec, i, 1, xh, y, s, e0, e1, true, best, &ms, reps));
         sec->tune_ms[pi][i] = ms;
         if (ms < best) {
             best = ms;
             best_i = i;
         }
     }
-    for (int c = 0; c < kNumChunks; ++c) {
+    for (int c = 0; c < kNumChunks && small; ++c) sec->tune_ms[pi][kNumShapes + c] = -1.0f;
+    for (int i = 0; i < kNumShapes && small; ++i) sec->tune_ms[pi][kNumShapes + kNumChunks + i] = -1.0f;
+    for (int c = 0; c < kNumChunks && !small; ++c) {
         float ms = 0;
         if (kChunks[c] == 1) ms = sec->tune_ms[pi][best_i];
         else SED_TRY(time_shape<R>(sec, best_i, kChunks[c], xh, y, s, e0, e1, false, best, &ms));
@@ -500,7 +519,7 @@ static int pick_shape(sed_sector* sec, const typename C2<R>::T* xh, typename C2<
     // the close shapes once more at the chosen run length (-1: not fitting or not close)
     const int shape_c = best_i;
     const float best1 = sec->tune_ms[pi][shape_c];
-    for (int i = 0; i < kNumShapes; ++i) {
+    for (int i = 0; i < kNumShapes && !small; ++i) {
         float ms = 0;
         if (!shape_fits<R>(sec, i) || !(sec->tune_ms[pi][i] <= 1.2f * best1)) {
             sec->tune_ms[pi][kNumShapes + kNumChunks + i] = -1.0f;
@@ -574,11 +593,11 @@ int launch_matvec_xhat(sed_sector* sec, const void* xhat, void* y, bool fp64, hi
 }
 
 // The same without the autotuner (no events, no host synchronisation): the shape an
-// earlier matvec or solve chose for this precision, else shape 1 at run length 1.
+// earlier matvec or solve chose for this precision, else the default shape at run length 1.
 int launch_matvec_xhat_fixed(sed_sector* sec, const void* xhat, void* y, bool fp64, hipStream_t s) {
     if (sec->dim == 0) return SED_OK;
     const int pi = fp64 ? 0 : 1;
-    const int shape = sec->mv_shape[pi] >= 0 ? sec->mv_shape[pi] : 1;
+    const int shape = sec->mv_shape[pi] >= 0 ? sec->mv_shape[pi] : kDefaultShape;
     const int chunk = sec->mv_shape[pi] >= 0 ? sec->mv_chunk[pi] : 1;
     if (fp64)
         return run_any_shape<double>(sec, shape, chunk, reinterpret_cast<const double2*>(xhat),
