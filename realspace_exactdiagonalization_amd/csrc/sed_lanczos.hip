@@ -188,16 +188,18 @@ __global__ void __launch_bounds__(kRedBlock) k_reorth_check(const int* __restric
                                                             const double* __restrict__ alpha,
                                                             const double* __restrict__ beta) {
     __shared__ double red[kRedBlock / kWave];
+    __shared__ int sh_need;
     double acc = 0.0;
     for (int b = threadIdx.x; b < G; b += kRedBlock) acc += normpart[b];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, kWave);
     if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
     __syncthreads();
-    if (threadIdx.x != 0) return;
     double nrm2 = 0.0;
-    for (int k = 0; k < kRedBlock / kWave; ++k) nrm2 += red[k];
+    for (int k = 0; k < kRedBlock / kWave; ++k) nrm2 += red[k];   // the same order on every thread
+    __syncthreads();
     const int j = *step;
-    const bool need = pro_decide(pro, j, nrm2, h1[j + 1].x, h1[j].x, alpha, beta);
+    const bool need = pro_decide_wg<kRedBlock>(pro, j, nrm2, h1[j + 1].x, h1[j].x, alpha, beta, red, &sh_need);
+    if (threadIdx.x != 0) return;
     *gate = need ? 1 : 0;
     if (!need) h2[j] = make_double2(0.0, 0.0);
 }
@@ -264,9 +266,7 @@ __global__ void __launch_bounds__(kTailThreads) k_step_tail(T* __restrict__ V, i
     const double nw = sh_nrm;
     double a = sh_h[j].x;
     double nw2 = block_update<T, NT>(V + (int64_t)lo * n, n, nv - lo, sh_h + lo, w, sh_redd);
-    if (threadIdx.x == 0) sh_need = pro_decide(pro, j, nw2, nw, a, alpha, beta) ? 1 : 0;
-    __syncthreads();
-    if (sh_need) {
+    if (pro_decide_wg<NT>(pro, j, nw2, nw, a, alpha, beta, sh_redd, &sh_need)) {
         block_dots<T, NT>(V, n, nv, w, sh_h, &sh_nrm, sh_red);
         a += sh_h[j].x;
         nw2 = block_update<T, NT>(V, n, nv, sh_h, w, sh_redd);

@@ -168,9 +168,7 @@ __global__ void __launch_bounds__(kPackBlock) k_pack_lanczos(const PackItem* __r
         const double nw = sh_nrm;
         double a = sh_h[j].x;
         double nw2 = block_update<T, NT>(V + (int64_t)lo * n, n, nv - lo, sh_h + lo, w, sh_redd);
-        if (threadIdx.x == 0) sh_need = pro_decide(pro, j, nw2, nw, a, alpha, beta) ? 1 : 0;
-        __syncthreads();
-        if (sh_need) {
+        if (pro_decide_wg<NT>(pro, j, nw2, nw, a, alpha, beta, sh_redd, &sh_need)) {
             block_dots<T, NT>(V, n, nv, w, sh_h, &sh_nrm, sh_red);
             a += sh_h[j].x;
             nw2 = block_update<T, NT>(V, n, nv, sh_h, w, sh_redd);

@@ -149,6 +149,77 @@ __host__ __device__ inline bool pro_decide(ProState* pro, int j, double nrm2, do
     return need;
 }
 
+// pro_decide spread over the NT threads of one workgroup, all of which call it with the
+// same arguments: each k of the omega recurrence runs on its own thread with pro_decide's
+// arithmetic, and the maximum over k is order-free (fmax), so the state left and the
+// decision returned (to every thread) are pro_decide's, bit for bit.  The serial form
+// is a chain of dependent loads of alpha, beta, omega and the arrowhead on one thread.
+// red: NT / kWave doubles of shared memory; sh: one shared int.
+template <int NT>
+__device__ inline bool pro_decide_wg(ProState* pro, int j, double nrm2, double nw, double aj, const double* alpha,
+                                     const double* beta, double* red, int* sh) {
+    const int lo = pro->lo;
+    const int force = pro->force;
+    const bool need0 = !(nrm2 > (lo == 0 ? kReorthEta2Whole : kReorthEta2) * nw) || force > 0;
+    double* wn = pro->omega[(j + 1) % 3];
+    const double eps = pro->eps;
+    const bool rec = pro->enabled && lo > 0 && !need0;
+    double mx = 0.0;
+    if (rec) {
+        const double* wj = pro->omega[j % 3];
+        const double* wm = pro->omega[(j + 2) % 3];   // row j - 1
+        const int kk = pro->kk;
+        const double bj = sqrt(nrm2);
+        const double bjm = beta[j - 1];
+        const double noise = eps * fmax(pro->anorm, fabs(aj) + bj + bjm);
+        for (int k = threadIdx.x; k <= j - 2; k += NT) {
+            double tw = alpha[k] * wj[k];
+            if (k < kk) {
+                tw += pro->arrow[k] * wj[kk];
+            } else {
+                tw += beta[k] * wj[k + 1];
+                if (k > kk) {
+                    tw += beta[k - 1] * wj[k - 1];
+                } else {
+                    for (int b = 0; b < kk; ++b) tw += pro->arrow[b] * wj[b];
+                }
+            }
+            double x = tw - bjm * wm[k] - aj * wj[k];
+            x += x < 0.0 ? -noise : noise;
+            x /= bj;
+            wn[k] = x;
+            mx = fmax(mx, fabs(x));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int k = 0; k < NT / kWave; ++k) m = fmax(m, red[k]);
+        bool need = need0;
+        if (force > 0) --pro->force;
+        if (rec) {
+            if (!(m <= pro->delta)) {   // NaN (bj = 0) included
+                need = true;
+                pro->force = pro->force > 1 ? pro->force : 1;
+            } else {
+                wn[j - 1] = eps;
+                wn[j] = eps;
+            }
+        }
+        if (need || lo == 0) ++pro->nfull;
+        wn[j + 1] = 1.0;
+        *sh = need ? 1 : 0;
+    }
+    __syncthreads();
+    const bool need = *sh != 0;
+    if (need || lo == 0)
+        for (int k = threadIdx.x; k <= j; k += NT) wn[k] = eps;
+    __syncthreads();
+    return need;
+}
+
 // End of step j (alpha_j = a, beta_j = b final): the next step's first pass covers
 // v_j, v_{j+1} (the whole basis when partial re-orthogonalisation is off).  One thread.
 __host__ __device__ inline void pro_advance(ProState* pro, int j, double a, double b, const double* beta) {
