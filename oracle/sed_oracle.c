@@ -18,8 +18,10 @@
  *   _is_orbit_compatible / sector     :534-556
  *   get_canonical / project_to_sector :584-594, :627-640
  *   hopping_phase_for_stats           :646-654
- *   build_matrixfree_projection_table :680-728
- *   _apply_table!                     :731-765 (per-thread buffers, static split)
+ *   build_matrixfree_projection_table :680-728 (also over a sample of the columns,
+ *                                      for the host baseline's bounded timing)
+ *   _apply_table!                     :731-765 (per-thread buffers kept between calls,
+ *                                      zeroed per call, static split; optionally timed by part)
  *   apply_hamiltonian!                :813-878 (on-the-fly, threaded over columns)
  *   update_orbit_stabilizer_phases!   :338-369
  *   _build_density_correlation_matrix! observables/static_structure_factor.jl:41-104
@@ -47,6 +49,9 @@
  *   phase refresh                      equal to a catalog rebuilt at the new phases
  *   sampled rows, basis checker        equal to the projection table on small sectors; planted
  *                                      defects in a basis are each reported
+ *   table of sampled columns           exactly those columns' entries of the whole table, in
+ *                                      order; _apply_table! over it equals the on-the-fly
+ *                                      matvec on those columns
  *
  * Complex products are evaluated as the reference writes them, left to right;
  * build with -ffp-contract=off so no FMA contraction changes the rounding.
