@@ -18,7 +18,10 @@
 //     D[i][j] in lane j + 16 (i % 4), register i / 4.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
+
+#include <hip/hip_runtime.h>
 
 #include "sed_device.hpp"
 #include "sed_vec.hpp"
@@ -295,57 +298,79 @@ __global__ void __launch_bounds__(256) k_orth_update_mfma(T* V0, int64_t n, cons
 
 // In place, V_i[r] <- sum_{j<m} V_j[r] S[j * kk + i] for i < kk (S real, m <= kMaxBasis).
 //
-// 16x16x4 MFMA over slabs of 8 complex rows (16 reals): A[e][k] = real pi(e) of
-// V_{4t+k} in the slab, B[k][i] = S[4t+k][16 tau + i], D[e][i] = real pi(e) of output
-// vector 16 tau + i, with pi(e) = 4 (e % 4) + e / 4 so that lane l holds the reals
-// 4 (l >> 4) .. 4 (l >> 4) + 3 (two complex rows) of output 16 tau + (l & 15).  A
-// wave loads all m vectors of its slab before it writes any output.
-template <typename T>
+// 16x16x4 MFMA over slabs of 16 complex rows, U slabs per wave and iteration.  Lane
+// e + 16 k loads the whole complex V_{4t+k}[s0 + e] (one 16- or 8-byte load: a load
+// instruction covers 16 consecutive rows of 4 vectors), and each step t runs two MFMAs on
+// the same B[k][i] = S[4t+k][16 tau + i]: A[e][k] = Re V_{4t+k}[s0 + e] and the imaginary
+// parts, so D[e][i] is the real (imaginary) part of output 16 tau + i at row s0 + e; lane
+// l then holds rows s0 + (l >> 4) + 4 r (register r) of output 16 tau + (l & 15) and
+// stores both parts as one complex.  NS = steps of the instantiation (>= ceil(m / 4));
+// vectors j >= m are loaded clamped to vector m - 1 (addresses already read, so cached)
+// and enter as zeros, rows >= n clamped to row n - 1 and never stored.  Each output
+// element accumulates the same products in the same order whatever NS and U are.  A wave
+// loads all m vectors of its slabs before it writes any output; S comes in dynamic LDS
+// (m * kk doubles).
+template <typename T, int NS, int U>
 __global__ void __launch_bounds__(256) k_rotate_mfma(T* V, int64_t ld, int m, const double* __restrict__ S, int kk,
                                                      int64_t n) {
-    constexpr int kSteps = kMaxBasis / 4;
+    extern __shared__ double lS[];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int e = lane & 15, k = lane >> 4;
-    const int pe = 4 * (e & 3) + (e >> 2);          // real index within the slab
-    const int nsteps = (m + 3) / 4;
     const int ntile = (kk + 15) / 16;
-    const int64_t stride = (int64_t)gridDim.x * (256 / kWave) * 8;
-    __shared__ double lS[kMaxBasis * kMaxBasis];
+    const int64_t stride = (int64_t)gridDim.x * (256 / kWave) * 16 * U;
     for (int i = threadIdx.x; i < m * kk; i += blockDim.x) lS[i] = S[i];
     __syncthreads();
-    for (int64_t s0 = ((int64_t)blockIdx.x * (256 / kWave) + wv) * 8; s0 < n; s0 += stride) {
-        double a[kSteps];
-        const int64_t ra = s0 + (pe >> 1);
+    for (int64_t s0 = ((int64_t)blockIdx.x * (256 / kWave) + wv) * 16 * U; s0 < n; s0 += stride) {
+        double2 a[U][NS];
 #pragma unroll
-        for (int t = 0; t < kSteps; ++t) {
-            const int j = 4 * t + k;
-            double x = 0.0;
-            if (t < nsteps && j < m && ra < n) {
-                const double2 v = to_d2(V[(int64_t)j * ld + ra]);
-                x = (pe & 1) ? v.y : v.x;
+        for (int u = 0; u < U; ++u) {
+            const int64_t r = s0 + 16 * u + e;
+            const int64_t ra = r < n ? r : n - 1;
+#pragma unroll
+            for (int t = 0; t < NS; ++t) {
+                const int j = 4 * t + k;
+                a[u][t] = to_d2(V[(int64_t)(j < m ? j : m - 1) * ld + ra]);
             }
-            a[t] = x;
         }
         for (int tau = 0; tau < ntile; ++tau) {
-            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-            const int io = 16 * tau + e;              // output vector of the B column / D column
+            const int io = 16 * tau + e;   // output vector of the B column
 #pragma unroll
-            for (int t = 0; t < kSteps; ++t) {
-                if (t < nsteps) {
+            for (int u = 0; u < U; ++u) {
+                dbl4 ar = {0.0, 0.0, 0.0, 0.0}, ai = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int t = 0; t < NS; ++t) {
                     const int j = 4 * t + k;
-                    const double sv = (j < m && io < kk) ? lS[j * kk + io] : 0.0;
-                    acc = mfma16x16(a[t], sv, acc);
+                    const bool ok = j < m;
+                    const double sv = (ok && io < kk) ? lS[j * kk + io] : 0.0;
+                    ar = mfma16x16(ok ? a[u][t].x : 0.0, sv, ar);
+                    ai = mfma16x16(ok ? a[u][t].y : 0.0, sv, ai);
                 }
-            }
-            // lane l: reals 4 (l >> 4) + v of output 16 tau + (l & 15), i.e. complex rows
-            // s0 + 2 (l >> 4) and s0 + 2 (l >> 4) + 1
-            const int64_t rw = s0 + 2 * k;
-            if (io < kk) {
-                if (rw < n) V[(int64_t)io * ld + rw] = from_d2<T>(make_double2(acc[0], acc[1]));
-                if (rw + 1 < n) V[(int64_t)io * ld + rw + 1] = from_d2<T>(make_double2(acc[2], acc[3]));
+                if (io < kk) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int64_t rw = s0 + 16 * u + k + 4 * r;
+                        if (rw < n) V[(int64_t)io * ld + rw] = from_d2<T>(make_double2(ar[r], ai[r]));
+                    }
+                }
             }
         }
     }
+}
+
+// Launch of the rotation on stream st: the instantiation with the fewest steps that cover
+// m among 40, 48 and 64 vectors (two slabs per wave up to 40 vectors, one above: register
+// budget), one 4-wave block per 64 U rows up to 16384 blocks.
+template <typename T>
+inline hipError_t launch_rotate(hipStream_t st, T* V, int64_t ld, int m, const double* S, int kk, int64_t n) {
+    const size_t lds = (size_t)m * kk * sizeof(double);
+    auto go = [&](auto kern, int u) {
+        const unsigned g = (unsigned)std::min<int64_t>(16384, (n + 64 * u - 1) / (64 * u));
+        hipLaunchKernelGGL(kern, dim3(g), dim3(256), lds, st, V, ld, m, S, kk, n);
+        return hipGetLastError();
+    };
+    if (m <= 40) return go(k_rotate_mfma<T, 10, 2>, 2);
+    if (m <= 48) return go(k_rotate_mfma<T, 12, 1>, 1);
+    return go(k_rotate_mfma<T, 16, 1>, 1);
 }
 
 }  // namespace sed

@@ -930,9 +930,7 @@ class ThickLanczos {
     int ritz(int m, const std::vector<double>& S, int kk) {
         SED_HIP(hipMemcpyAsync(ws_.S.p, S.data(), S.size() * sizeof(double), hipMemcpyHostToDevice, st_));
         if (mfma_) {
-            const unsigned g = (unsigned)std::min<int64_t>(4096, (n_ + 31) / 32);
-            hipLaunchKernelGGL(k_rotate_mfma<T>, dim3(g), dim3(256), 0, st_, V_, (int64_t)n_, m, ws_.S.p, kk,
-                               (int64_t)n_);
+            SED_HIP(launch_rotate<T>(st_, V_, (int64_t)n_, m, ws_.S.p, kk, (int64_t)n_));
         } else {
             const size_t lds = (((size_t)m * kk * 8 + 15) & ~(size_t)15) + (size_t)m * kRitzRows * sizeof(T);
             const unsigned g = (unsigned)((n_ + kRitzRows - 1) / kRitzRows);

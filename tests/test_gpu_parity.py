@@ -528,6 +528,7 @@ PRO_CASES = {
     # name: (model, twist, irrep index, nev)
     "checker3x4_f": (lambda: L.checkerboard(3, 4, n_filled=8), (0.0, 0.0), 0, 4),
     "hald3x4_flux_nev8": (lambda: L.haldane_honeycomb(3, 4, n_filled=6), (0.3, 0.0), 5, 8),
+    "hald3x4_flux_nev10": (lambda: L.haldane_honeycomb(3, 4, n_filled=6), (0.3, 0.0), 4, 10),
     "hald3x4_flux_nev20": (lambda: L.haldane_honeycomb(3, 4, n_filled=6), (0.3, 0.0), 2, 20),
     "xxz20": (lambda: L.heisenberg_chain(20), (0.0, 0.0), 0, 3),
 }
@@ -585,8 +586,8 @@ def test_degenerate_spectrum_nev20(ed, ctx, oracle_mod, monkeypatch, mode):
 
 
 @pytest.mark.parametrize("name,reorth", [("checker3x4_f", "mfma"), ("checker3x4_f", None),
-                                         ("hald3x4_flux_nev8", None), ("hald3x4_flux_nev20", "mfma"),
-                                         ("xxz20", None)])
+                                         ("hald3x4_flux_nev8", None), ("hald3x4_flux_nev10", "mfma"),
+                                         ("hald3x4_flux_nev20", "mfma"), ("xxz20", None)])
 def test_partial_reorthogonalisation_matches_full(ed, ctx, oracle_mod, monkeypatch, name, reorth):
     """Partial re-orthogonalisation (the default: Simon's recurrence decides when a step is
     orthogonalised against the whole basis) against a pass over the whole basis at every
@@ -596,8 +597,9 @@ def test_partial_reorthogonalisation_matches_full(ed, ctx, oracle_mod, monkeypat
     orthogonalises against the whole basis), and bitwise reproducible runs.
     Sectors with several thick restarts: checkerboard 3x4 fermions (dim 61,333; also with
     the matrix-core kernels forced, SED_REORTH=mfma, which large sectors use by default),
-    the twisted Haldane 3x4 bosons with nev = 8 (dim ~11,200) and with nev = 20 (the
-    largest Krylov basis, m = 64, matrix-core kernels forced), XXZ N=20 (dim 9,252)."""
+    the twisted Haldane 3x4 bosons with nev = 8 (dim ~11,200), nev = 10 (m = 44) and nev = 20
+    (the largest Krylov basis, m = 64), the last two with the matrix-core kernels forced (the
+    restart rotation's three instantiations: m <= 40, <= 48, <= 64), XXZ N=20 (dim 9,252)."""
     if reorth:
         monkeypatch.setenv("SED_REORTH", reorth)
     f, th, k_idx, nev = PRO_CASES[name]
