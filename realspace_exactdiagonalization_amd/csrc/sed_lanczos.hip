@@ -179,6 +179,147 @@ __global__ void __launch_bounds__(256) k_update(T* V0, int64_t n, const int* __r
     }
 }
 
+// The row-wise kernels of a range of fewer than kMfmaMinVectors vectors beside the
+// matrix-core kernels (disp 2; the local pass v_{j-1}, v_j of most steps): the same sums
+// in the same order as k_dots / k_update (bitwise equal), with U rows of a thread in
+// flight: every load of U rows (w and kFew vectors, slots past the range re-reading
+// vector lo, which is cached) is issued before the first product.  Complex-FP32 vectors
+// otherwise keep too few bytes in flight per wave to stream at full rate.
+constexpr int kFew = kMfmaMinVectors - 1;
+
+template <typename T, int U>
+__global__ void __launch_bounds__(kRedBlock) k_dots_few(const T* __restrict__ V0, int64_t n,
+                                                        const int* __restrict__ step, const T* __restrict__ w0,
+                                                        double2* __restrict__ part0, const int* __restrict__ gate,
+                                                        const int* __restrict__ lo, int w_next) {
+    __shared__ double2 red[kFew + 1][kRedBlock / kWave];
+    if (gate && *gate == 0) return;
+    PassRange pr;
+    if (!pass_range(step, lo, 2, pr)) return;
+    const T* __restrict__ V = V0 + (int64_t)pr.lo * n;
+    const T* __restrict__ w = w_next ? V0 + (int64_t)(pr.j + 1) * n : w0;
+    double2* __restrict__ part = part0 + (int64_t)pr.lo * gridDim.x;
+    const int nv = pr.nv;
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = (int64_t)blockIdx.x * chunk;
+    const int64_t r1 = std::min<int64_t>(n, r0 + chunk);
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    double2 acc[kFew + 1];
+#pragma unroll
+    for (int q = 0; q <= kFew; ++q) acc[q] = make_double2(0.0, 0.0);
+    for (int64_t rb = r0 + threadIdx.x; rb < r1; rb += (int64_t)kRedBlock * U) {
+        T wr[U], vv[U][kFew];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t r = rb + (int64_t)u * kRedBlock;
+            const int64_t rc = r < r1 ? r : rb;
+            wr[u] = w[rc];
+#pragma unroll
+            for (int q = 0; q < kFew; ++q) vv[u][q] = V[(int64_t)(q < nv ? q : 0) * n + rc];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (rb + (int64_t)u * kRedBlock >= r1) break;
+            const double2 wd = to_d2(wr[u]);
+#pragma unroll
+            for (int q = 0; q <= kFew; ++q) {
+                if (q < nv) {
+                    const double2 v = to_d2(vv[u][q < kFew ? q : 0]);
+                    acc[q].x += v.x * wd.x + v.y * wd.y;
+                    acc[q].y += v.x * wd.y - v.y * wd.x;
+                } else if (q == nv) {
+                    acc[q].x += wd.x * wd.x + wd.y * wd.y;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q <= kFew; ++q) {
+        const double2 sm = wave_sum_d2(acc[q]);
+        if (lane == 0) red[q][wv] = sm;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x <= nv) {
+        double2 sm = make_double2(0.0, 0.0);
+        for (int i = 0; i < kRedBlock / kWave; ++i) {
+            sm.x += red[threadIdx.x][i].x;
+            sm.y += red[threadIdx.x][i].y;
+        }
+        part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = sm;
+    }
+}
+
+template <typename T, int U>
+__global__ void __launch_bounds__(256) k_update_few(T* V0, int64_t n, const int* __restrict__ step,
+                                                    const double2* __restrict__ h, T* w0, int to_next,
+                                                    double* __restrict__ normpart, const int* __restrict__ gate,
+                                                    const int* __restrict__ lo, int w_next) {
+    __shared__ double2 lh[kFew];
+    __shared__ double red[256 / kWave];
+    if (gate && *gate == 0) return;
+    PassRange pr;
+    if (!pass_range(step, lo, 2, pr)) return;
+    const int nv = pr.nv;
+    if ((int)threadIdx.x < nv) lh[threadIdx.x] = h[pr.lo + threadIdx.x];
+    __syncthreads();
+    const T* V = V0 + (int64_t)pr.lo * n;
+    T* out = to_next ? V0 + (int64_t)(pr.j + 1) * n : w0;
+    const T* w = w_next ? V0 + (int64_t)(pr.j + 1) * n : w0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    double nrm = 0.0;
+    for (int64_t rb = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; rb < n; rb += stride * U) {
+        T wr[U], vv[U][kFew];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t r = rb + u * stride;
+            const int64_t rc = r < n ? r : rb;
+            wr[u] = w[rc];
+#pragma unroll
+            for (int i = 0; i < kFew; ++i) vv[u][i] = V[(int64_t)(i < nv ? i : 0) * n + rc];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t r = rb + u * stride;
+            if (r >= n) break;
+            double2 a = to_d2(wr[u]);
+#pragma unroll
+            for (int i = 0; i < kFew; ++i) {
+                if (i < nv) {
+                    const double2 v = to_d2(vv[u][i]);
+                    const double2 c = lh[i];
+                    a.x -= v.x * c.x - v.y * c.y;
+                    a.y -= v.x * c.y + v.y * c.x;
+                }
+            }
+            const T o = from_d2<T>(a);
+            out[r] = o;
+            const double2 od = to_d2(o);
+            nrm += od.x * od.x + od.y * od.y;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) nrm += __shfl_down(nrm, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sm = 0.0;
+        for (int k = 0; k < 256 / kWave; ++k) sm += red[k];
+        normpart[blockIdx.x] = sm;
+    }
+}
+// rows per thread in flight: measured on a 15.7M-row sector (local pass, two vectors),
+// complex FP32 dots 127 -> 84 us and update 125 -> 100 us at two rows; complex FP64 dots
+// 147 -> 137 us at four rows, its update fastest as k_update (one row)
+template <typename T>
+struct FewRows;
+template <>
+struct FewRows<float2> {
+    static constexpr int dots = 2, update = 2;
+};
+template <>
+struct FewRows<double2> {
+    static constexpr int dots = 4, update = 1;
+};
+
 // Single block, after the first pass: |w'|^2 from the update's partials; pro_decide
 // sets the gate of the second pass (and clears h2[j] when it does not run).
 __global__ void __launch_bounds__(kRedBlock) k_reorth_check(const int* __restrict__ step,
@@ -838,18 +979,29 @@ class ThickLanczos {
     // itself (w_next = 1, in place).  With the matrix-core kernels both kernel families
     // are launched and the range size picks the one that works (pass_range).
     int launch_orth_pass(double2* h, const int* lo, int w_next, const int* gate) {
-        const int dm = mfma_ ? 1 : 0, dv = mfma_ ? 2 : 0;
-        if (mfma_)
+        if (mfma_) {
             hipLaunchKernelGGL(k_proj_mfma<T>, dim3(G_), dim3(256), 0, st_, V_, (int64_t)n_, step_, w_, part_, gate,
-                               lo, w_next, dm);
-        hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_, gate, lo,
-                           w_next, dv);
+                               lo, w_next, 1);
+            hipLaunchKernelGGL((k_dots_few<T, FewRows<T>::dots>), dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_,
+                               step_, w_, part_, gate, lo, w_next);
+        } else {
+            hipLaunchKernelGGL(k_dots<T>, dim3(G_), dim3(kRedBlock), 0, st_, V_, (int64_t)n_, step_, w_, part_, gate,
+                               lo, w_next, 0);
+        }
         hipLaunchKernelGGL(k_reduce, dim3(m_ + 2), dim3(kRedBlock), 0, st_, part_, G_, step_, h, gate, lo);
-        if (mfma_)
+        if (mfma_) {
             hipLaunchKernelGGL(k_orth_update_mfma<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h, w_,
-                               1, normpart_, gate, lo, w_next, dm);
-        hipLaunchKernelGGL(k_update<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h, w_, 1, normpart_,
-                           gate, lo, w_next, dv);
+                               1, normpart_, gate, lo, w_next, 1);
+            if constexpr (FewRows<T>::update > 1)
+                hipLaunchKernelGGL((k_update_few<T, FewRows<T>::update>), dim3(kUpdGrid), dim3(256), 0, st_, V_,
+                                   (int64_t)n_, step_, h, w_, 1, normpart_, gate, lo, w_next);
+            else
+                hipLaunchKernelGGL(k_update<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h, w_, 1,
+                                   normpart_, gate, lo, w_next, 2);
+        } else {
+            hipLaunchKernelGGL(k_update<T>, dim3(kUpdGrid), dim3(256), 0, st_, V_, (int64_t)n_, step_, h, w_, 1,
+                               normpart_, gate, lo, w_next, 0);
+        }
         SED_HIP(hipGetLastError());
         return SED_OK;
     }
